@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark of the AdaptiveKSearchSurface hot path on one MI355X.
+
+Workload: terrestrial-scan-like room, 4,000,000 float32 points ray-cast from a scanner at the
+origin (3200 x 1250 angular grid) into a 20 x 15 x 4 m room with 40 boxes/cylinders, radial
+noise 2 mm; kn=50, ki=20, kd=10, flip point = scanner origin, voxel 0.05 m.  The SDF query
+vertices are the PointsetGrid lattice of that cloud (cells touched by points, extruded by one
+cell, all cell corners).
+
+A step = calculateNormals (adaptive kNN + PCA + flip) + interpolateNormals + distances at all
+query vertices, inputs resident in HBM.  value = points / (t_normals + t_interpolate) in
+Mpts/s; the SDF rate is reported beside it.  The roofline entry covers the dominant kernel
+(fused kNN + PCA, first adaptive round), timed with hipEvents on the library's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def grid_vertices(points_t, voxel: float, extrude: bool = True):
+    """PointsetGrid vertex set (PointsetGrid.tcc:41-109, HashGrid::addBox) on the GPU with torch:
+    cells floor(p / voxel) whose centre lies in the cloud's box, optionally extruded by the 26
+    neighbours, then every cell corner once, at centre - voxel/2 per axis of its cell."""
+    import torch
+
+    dev = points_t.device
+    vs = torch.tensor(voxel, dtype=torch.float32, device=dev)
+    lo = points_t.min(0).values
+    hi = points_t.max(0).values
+    off = 1 << 20
+
+    def pack(c):
+        c = c + off
+        return (c[:, 0] << 42) | (c[:, 1] << 21) | c[:, 2]
+
+    def unpack(k):
+        return torch.stack([(k >> 42) - off, ((k >> 21) & ((1 << 21) - 1)) - off, (k & ((1 << 21) - 1)) - off], 1)
+
+    def centre_inside(c):
+        ctr = (c.to(torch.float32) + 0.5) * vs
+        return ((ctr >= lo) & (ctr <= hi)).all(1)
+
+    cells = torch.floor(points_t / vs).to(torch.int64)
+    cells = cells[centre_inside(cells)]
+    keys = torch.unique(pack(cells))
+    if extrude:
+        c = unpack(keys)
+        d = torch.stack(torch.meshgrid(*(torch.arange(-1, 2, device=dev),) * 3, indexing="ij"), -1).reshape(-1, 3)
+        allc = (c[:, None, :] + d[None]).reshape(-1, 3)
+        allc = allc[centre_inside(allc)]
+        keys = torch.unique(pack(allc))
+        del allc
+    c = unpack(keys)
+    corners = torch.stack(torch.meshgrid(*(torch.arange(0, 2, device=dev),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    vk = torch.unique(pack((c[:, None, :] + corners[None]).reshape(-1, 3)))
+    lat = unpack(vk)
+    # the corner shared by cells c..c-1 sits at ((c-1) + 0.5) * voxel + voxel/2
+    pos = ((lat - 1).to(torch.float32) + 0.5) * vs + vs / 2
+    return pos.contiguous(), int(keys.numel())
+
+
+def cpu_baseline(w, gpu_normals, verts, sample_pts: int, sample_q: int):
+    """The oracle (CPU restatement of the reference path, OpenMP over all host cores) on a
+    bounded sample of the same workload."""
+    import oracle
+
+    if not os.path.exists(oracle.LIB_PATH):
+        oracle.build()
+    rng = np.random.default_rng(0)
+    t = time.perf_counter()
+    o = oracle.Surface(w.points)
+    t_build = time.perf_counter() - t
+    sub = np.sort(rng.choice(len(w.points), sample_pts, replace=False))
+    t = time.perf_counter()
+    o.normals(w.kn, 3, w.flip, subset=sub)
+    t_norm = time.perf_counter() - t
+    t = time.perf_counter()
+    o.interpolate(gpu_normals, w.ki, subset=sub)
+    t_int = time.perf_counter() - t
+    qs = verts[np.sort(rng.choice(len(verts), min(sample_q, len(verts)), replace=False))]
+    t = time.perf_counter()
+    o.distance(gpu_normals, w.kd, qs)
+    t_sdf = time.perf_counter() - t
+    return {
+        "mpts": sample_pts / (t_norm + t_int) / 1e6,
+        "sdf_mqps": len(qs) / t_sdf / 1e6,
+        "cores": oracle.num_threads(),
+        "t_build_s": t_build,
+        "t_norm_s": t_norm,
+        "t_interp_s": t_int,
+        "t_sdf_s": t_sdf,
+        "sample": f"{sample_pts} random points of the {len(w.points)}-point room (normals kn={w.kn} + "
+                  f"interpolation ki={w.ki}, kd-tree over all points) and {len(qs)} random lattice vertices "
+                  f"(SDF kd={w.kd})",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-az", type=int, default=3200)
+    ap.add_argument("--n-el", type=int, default=1250)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-sample-q", type=int, default=300_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from lvr2_amd import HipSurface, scenes
+
+    log("[bench] generating scene")
+    w = scenes.room(args.n_az, args.n_el)
+    n = len(w.points)
+    dev = torch.device("cuda:0")
+    pts_t = torch.from_numpy(w.points).to(dev)
+    st = torch.cuda.current_stream(dev)
+    log(f"[bench] {n} points; building lattice")
+    verts_t, ncells = grid_vertices(pts_t, w.voxel)
+    m = int(verts_t.shape[0])
+    log(f"[bench] {m} query vertices ({ncells} cells); building LBVH")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    s = HipSurface(pts_t)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t) * 1e3
+    s.setKn(w.kn)
+    s.setKi(w.ki)
+    s.setKd(w.kd)
+    s.setFlippoint(*w.flip)
+    out = (torch.empty(m, dtype=torch.float32, device=dev), torch.empty(m, dtype=torch.float32, device=dev),
+           torch.empty(m, dtype=torch.uint8, device=dev))
+
+    def step(evs=None):
+        if evs:
+            evs[0].record(st)
+        s.calculateNormals()
+        if evs:
+            evs[1].record(st)
+        s.interpolateNormals()
+        if evs:
+            evs[2].record(st)
+        s.distances(verts_t, w.voxel, out)
+        if evs:
+            evs[3].record(st)
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    s.setProfiling(True)
+    s.timings(reset=True)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(evs[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"[bench] {args.steps} timed steps done in {wall:.3f} s")
+    t_norm = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
+    t_int = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+    t_sdf = sum(e[2].elapsed_time(e[3]) for e in evs) / args.steps
+    ms_step = sum(e[0].elapsed_time(e[3]) for e in evs) / args.steps
+    tm = s.timings()
+    rounds = s.normalsStats()
+    mpts = n / ((t_norm + t_int) / 1e3) / 1e6
+    sdf_mqps = m / (t_sdf / 1e3) / 1e6
+
+    # roofline of the dominant kernel: fused kNN + PCA, first adaptive round (k = 2 kn)
+    k1 = 2 * w.kn
+    bytes_per_pt = 12 + 12 * k1 + 12 + 4 * w.ki
+    ms_first, n_first = tm["normals_first"]
+    avg_ms = ms_first / max(n_first, 1)
+    achieved = n * bytes_per_pt / (avg_ms / 1e3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": "knn_tile_kernel<4, SinkNormals<4>> (first adaptive round)",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(avg_ms, 4),
+        "bytes_per_unit": bytes_per_pt,
+        "units_per_launch": n,
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_knn_pca_room_4m.json")
+    if os.path.exists(traffic_file):
+        try:
+            roofline["traffic"] = json.load(open(traffic_file)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            pass
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle, bounded sample)")
+        gn = s.getNormals()
+        cpu = cpu_baseline(w, gn, verts_t.cpu().numpy(), args.cpu_sample, args.cpu_sample_q)
+        log(f"[bench] CPU baseline done: {cpu['mpts']:.3f} Mpts/s, {cpu['sdf_mqps']:.3f} M q/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "Mpts/s kNN+PCA normals and M SDF queries/s at kn=50, vs host CPU; % HBM peak",
+        "value": round(mpts, 2),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ray-cast room scan)",
+        "config": {"workload": f"room scan {n} pts kn50 ki20 kd10 voxel 0.05 m", "points": n, "queries": m,
+                   "kn": w.kn, "ki": w.ki, "kd": w.kd, "voxel": w.voxel},
+        "sdf_mqps": round(sdf_mqps, 2),
+        "phase_ms": {"normals": round(t_norm, 3), "interpolate": round(t_int, 3), "sdf": round(t_sdf, 3),
+                     "build": round(build_ms, 2)},
+        "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in tm.items() if v[1]},
+        "adaptive_rounds": [int(x) for x in rounds],
+        "roofline": roofline,
+        "cpu_baseline": None if cpu is None else {
+            "value": round(cpu["mpts"], 4), "unit": "Mpts/s", "cores": cpu["cores"], "kind": "port",
+            "sample": cpu["sample"], "sdf_mqps": round(cpu["sdf_mqps"], 4),
+        },
+    }
+    if cpu:
+        line["speedup_vs_cpu"] = {"normals": round(mpts / cpu["mpts"], 1), "sdf": round(sdf_mqps / cpu["sdf_mqps"], 1)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,152 @@
+/*
+ * lvrhip — C ABI of the MI355X (gfx950) implementation of lvr2's
+ * AdaptiveKSearchSurface hot path: LBVH build, exact kNN, PCA normals with adaptive
+ * neighbourhoods, normal interpolation and the Hoppe signed distance at grid vertices.
+ *
+ * Conventions
+ *  - Every function returns 0 on success, otherwise an LVRHIP_E* code;
+ *    lvrhip_last_error() returns a thread-local message for the last failure.  For a
+ *    device allocation failure the message contains "out of memory", the substring lvr2's
+ *    LargeScaleReconstruction matches (LargeScaleReconstruction.tcc:656-662).
+ *  - Host pointers are caller-owned.  Device state is owned by the handle.
+ *  - Point arrays are AoS float32 xyz (n*3 floats), like lvr2's "points" channel
+ *    (types/Channel.hpp:43-84).  Indices are 0-based positions in that array.
+ *  - kNN order: the k smallest keys (fp32 d^2, point index), where
+ *    d^2 = (dx*dx + dy*dy) + dz*dz is evaluated without FMA contraction.
+ *  - All work is issued on the handle's stream (default: the null stream); functions
+ *    that return host data synchronise that stream.
+ */
+#ifndef LVRHIP_H
+#define LVRHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lvrhip_surface* lvrhip_t;
+
+enum {
+    LVRHIP_OK = 0,
+    LVRHIP_EINVAL = 1,      /* bad argument */
+    LVRHIP_ENOMEM = 2,      /* device allocation failed ("out of memory") */
+    LVRHIP_EHIP = 3,        /* other HIP runtime error */
+    LVRHIP_ESTATE = 4,      /* call order violated (e.g. distances before normals) */
+    LVRHIP_ECAPACITY = 5    /* an internal fixed-size structure overflowed */
+};
+
+/* Normal estimation methods, numbered like AdaptiveKSearchSurface's calcMethod
+ * (AdaptiveKSearchSurface.hpp:289-293; dispatch at AdaptiveKSearchSurface.tcc:225-246). */
+enum {
+    LVRHIP_METHOD_LS_PLANE = 0,   /* calcPlane, tcc:467-520 */
+    LVRHIP_METHOD_ITERATIVE = 2,  /* calcPlaneIterative, tcc:522-607 */
+    LVRHIP_METHOD_PCA = 3         /* calcPlaneIPCAExact, tcc:609-673 */
+};
+
+const char* lvrhip_last_error(void);
+
+/* Replaces CudaSurface(floatArr& points, size_t num_points, int device)
+ * (cuda/CudaSurface.hpp:86; CudaSurface.cu:1039-1052) and the search-tree construction of
+ * AdaptiveKSearchSurface (AdaptiveKSearchSurface.tcc:76).  Copies the n points to the
+ * device and builds the LBVH.  Defaults follow CudaSurface::init (CudaSurface.cu:1007-1023):
+ * kn=10, ki=10, kd=5, flip point (1e6,1e6,1e6), method PCA. */
+int lvrhip_create(const float* xyz, size_t n, int device, lvrhip_t* out);
+
+/* Same, for points already resident in device memory (n*3 floats on `device`).
+ * The pointer is only read during the call. */
+int lvrhip_create_device(const float* d_xyz, size_t n, int device, void* stream, lvrhip_t* out);
+
+/* Replaces ~CudaSurface / CudaSurface::freeGPU (CudaSurface.cu:1259-1264). */
+int lvrhip_destroy(lvrhip_t h);
+
+/* Replaces CudaSurface::setKn/setKi/setKd (CudaSurface.hpp:116-142) and
+ * PointsetSurface::setKn/setKi/setKd (PointsetSurface.hpp:112-126).  ki == 0 disables
+ * interpolation like PointsetSurface. */
+int lvrhip_set_k(lvrhip_t h, int kn, int ki, int kd);
+
+/* Replaces CudaSurface::setFlippoint (CudaSurface.hpp:153) and
+ * PointsetSurface::setFlipPoint (PointsetSurface.hpp:128-131). */
+int lvrhip_set_flip_point(lvrhip_t h, float x, float y, float z);
+
+/* Replaces CudaSurface::setMethod (CudaSurface.cu:1200-1216); takes the
+ * AdaptiveKSearchSurface method number instead of a string. */
+int lvrhip_set_method(lvrhip_t h, int method);
+
+/* HIP stream for all subsequent work (hipStream_t passed as void*; NULL = null stream).
+ * No counterpart in lvr2, which always uses the default stream. */
+int lvrhip_set_stream(lvrhip_t h, void* stream);
+
+/* Replaces CudaSurface::calculateNormals (CudaSurface.cu:1238-1253) with the semantics of
+ * AdaptiveKSearchSurface::calculateSurfaceNormals without its interpolation tail
+ * (AdaptiveKSearchSurface.tcc:158-284): adaptive k = 2kn, 4kn, ... 32kn until the
+ * neighbourhood bounding box passes boundingBoxOK (tcc:337-350), the plane fit of the
+ * selected method, then the flip towards the flip point. */
+int lvrhip_calculate_normals(lvrhip_t h);
+
+/* Replaces CudaSurface::interpolateNormals (CudaSurface.cu:1255-1257, empty there) with
+ * AdaptiveKSearchSurface::interpolateSurfaceNormals (tcc:293-335). No-op when ki == 0. */
+int lvrhip_interpolate_normals(lvrhip_t h);
+
+/* Use the given normals (n*3 floats, AoS, input order) instead of estimating them: the
+ * "given normals" path of lvr2_reconstruct (src/tools/lvr2_reconstruct/Main.cpp:353,
+ * 385-388), where a point buffer that already has a "normals" channel skips
+ * calculateSurfaceNormals.  Enables interpolate_normals and distances. */
+int lvrhip_set_normals(lvrhip_t h, const float* normals_xyz);
+
+/* Replaces CudaSurface::getNormals(floatArr) (CudaSurface.hpp:72-80). out: n*3 floats. */
+int lvrhip_get_normals(lvrhip_t h, float* out_xyz);
+
+/* Device-resident variant: copies the normals (n*3 floats, AoS, input order) into device
+ * memory `d_out` on the handle's stream without synchronising. */
+int lvrhip_get_normals_device(lvrhip_t h, float* d_out);
+
+/* Replaces SearchTree::kSearch (SearchTree.hpp:49-70; SearchKDTree::kSearch,
+ * KDTree.tcc:331-348) for m queries at once.  out_idx/out_d2: m*k entries, ascending;
+ * rows are padded with -1 / +inf when the cloud has fewer than k points.  out_d2 may be
+ * NULL. */
+int lvrhip_knn(lvrhip_t h, const float* q, size_t m, int k, int32_t* out_idx, float* out_d2);
+
+/* Replaces CudaSurface::distances (CudaSurface.cu:1273-1317) with the semantics of
+ * AdaptiveKSearchSurface::distance (tcc:414-457) and the invalid rule of
+ * PointsetGrid::calcDistanceValues (PointsetGrid.tcc:138-143): for each of the m
+ * positions, proj = (v - mean_kd(p)) . normalize(mean_kd(n)), eucl = |v - mean_kd(p)|,
+ * invalid = eucl > 1.75 * voxel.  Requires normals (calculate_normals first).
+ * out_proj / out_eucl / out_invalid may individually be NULL. */
+int lvrhip_distances(lvrhip_t h, const float* q, size_t m, float voxel, float* out_proj, float* out_eucl,
+                     uint8_t* out_invalid);
+
+/* Device-resident variant of lvrhip_distances: all pointers are device memory; runs on the
+ * handle's stream without synchronising. */
+int lvrhip_distances_device(lvrhip_t h, const float* d_q, size_t m, float voxel, float* d_proj, float* d_eucl,
+                            uint8_t* d_invalid);
+
+/* Statistics of the last calculate_normals call: n_round[r] = number of points whose
+ * neighbourhood search ran with k = kn * 2^(r+1), r = 0..4.  No lvr2 counterpart. */
+int lvrhip_normals_stats(lvrhip_t h, int64_t n_round[5]);
+
+/* Number of points of the handle. */
+int lvrhip_size(lvrhip_t h, size_t* n);
+
+/* Device-time accounting, no lvr2 counterpart.  When enabled, the library brackets its
+ * kernels with hipEvents on the handle's stream and accumulates, per timer, the number of
+ * launches and the summed elapsed milliseconds (read after the call that launched them
+ * returns).  Timers: */
+enum {
+    LVRHIP_T_NORMALS_FIRST = 0, /* fused kNN + plane fit kernel, first adaptive round */
+    LVRHIP_T_NORMALS_RETRY = 1, /* later adaptive rounds incl. their list compaction */
+    LVRHIP_T_INTERPOLATE = 2,   /* interpolation kernel */
+    LVRHIP_T_SDF_KERNEL = 3,    /* fused kNN + signed distance kernel */
+    LVRHIP_T_SDF_SORT = 4,      /* Morton sort of the query vertices */
+    LVRHIP_T_BUILD = 5,         /* LBVH build */
+    LVRHIP_T_COUNT = 6
+};
+int lvrhip_set_profiling(lvrhip_t h, int on);
+/* ms[i], launches[i] for i < LVRHIP_T_COUNT; either may be NULL.  reset != 0 clears them. */
+int lvrhip_get_timings(lvrhip_t h, double* ms, int64_t* launches, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+"""ctypes binding of liblvrhip.so (declarations: include/lvrhip.h).
+
+The library is required: importing the binding without a built ``liblvrhip.so`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblvrhip.so")
+
+# exported symbols and their ctypes signatures
+_SIGS = {
+    "lvrhip_last_error": (C.c_char_p, []),
+    "lvrhip_create": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
+    "lvrhip_create_device": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "lvrhip_destroy": (C.c_int, [C.c_void_p]),
+    "lvrhip_set_k": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "lvrhip_set_flip_point": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+    "lvrhip_set_method": (C.c_int, [C.c_void_p, C.c_int]),
+    "lvrhip_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lvrhip_calculate_normals": (C.c_int, [C.c_void_p]),
+    "lvrhip_interpolate_normals": (C.c_int, [C.c_void_p]),
+    "lvrhip_set_normals": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lvrhip_get_normals": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lvrhip_get_normals_device": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lvrhip_knn": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p]),
+    "lvrhip_distances": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_float, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]),
+    "lvrhip_distances_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_float, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
+    "lvrhip_normals_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "lvrhip_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "lvrhip_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "lvrhip_get_timings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+}
+
+TIMERS = ("normals_first", "normals_retry", "interpolate", "sdf_kernel", "sdf_sort", "build")
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+class LvrHipError(RuntimeError):
+    """Error returned by the C ABI; ``code`` is the LVRHIP_E* value."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m lvr2_amd.build`")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().lvrhip_last_error().decode(errors="replace")
+        raise LvrHipError(rc, msg)

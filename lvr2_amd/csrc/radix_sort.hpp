@@ -1,0 +1,161 @@
+// LSD radix sort of (u64 key, u32 value) pairs and an exclusive scan, hand-written for
+// gfx950.  Replaces the host-side thrust::sort_by_key of LBVHIndex::build
+// (src/liblvr2/reconstruction/cuda/LBVHIndex.cu:138, 532).
+//
+// One pass = 8 key bits: (1) per-tile digit histograms, (2) one exclusive scan over the
+// digit-major histogram table, (3) a stable scatter in which each 256-thread block ranks
+// its 4096 keys with wave-level match masks (8 ballots per key) and per-wave digit counts.
+#pragma once
+
+#include "common.hpp"
+
+namespace lvrhip {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_RADIX = 256;
+
+__global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                              uint32_t* __restrict__ ghist, int ntiles)
+{
+    __shared__ uint32_t h[RS_RADIX];
+    const int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t i = base + (int64_t)r * RS_THREADS + tid;
+        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    ghist[(int64_t)tid * ntiles + blockIdx.x] = h[tid];
+}
+
+// In-place exclusive scan of a[0..len) with one 1024-thread block, 4096 elements per step.
+__global__ __launch_bounds__(1024) void scan_exclusive_kernel(uint32_t* __restrict__ a, int64_t len,
+                                                              uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry_s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry_s = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < len; base += 4096) {
+        uint32_t v[4];
+        uint32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int64_t i = base + (int64_t)tid * 4 + j;
+            v[j] = i < len ? a[i] : 0u;
+            s += v[j];
+        }
+        // inclusive wave scan of per-thread sums
+        uint32_t incl = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t wpre = 0, tot = 0;
+        for (int w = 0; w < 16; w++) {
+            uint32_t x = wsum[w];
+            if (w < wave) wpre += x;
+            tot += x;
+        }
+        const uint32_t carry = carry_s;
+        uint32_t run = carry + wpre + incl - s;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int64_t i = base + (int64_t)tid * 4 + j;
+            if (i < len) a[i] = run;
+            run += v[j];
+        }
+        __syncthreads();
+        if (tid == 0) carry_s = carry + tot;
+        __syncthreads();
+    }
+    if (tid == 0 && total) *total = carry_s;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t* __restrict__ kin,
+                                                                 const uint32_t* __restrict__ vin,
+                                                                 uint64_t* __restrict__ kout,
+                                                                 uint32_t* __restrict__ vout, int64_t n, int shift,
+                                                                 const uint32_t* __restrict__ goff, int ntiles)
+{
+    __shared__ uint32_t off[RS_RADIX];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][RS_RADIX];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    off[tid] = goff[(int64_t)tid * ntiles + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][tid] = 0;
+    __syncthreads();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const int64_t i = base + (int64_t)r * RS_THREADS + tid;
+        const bool valid = i < n;
+        uint64_t key = valid ? kin[i] : 0ull;
+        uint32_t val = valid ? vin[i] : 0u;
+        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bal = ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const int rank = __popcll(peers & lt_mask);
+        if (valid && (peers & lt_mask) == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pre = off[d] + (uint32_t)rank;
+            for (int w = 0; w < wave; w++) pre += wcnt[w][d];
+            kout[pre] = key;
+            vout[pre] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; w++) {
+            add += wcnt[w][tid];
+            wcnt[w][tid] = 0;
+        }
+        off[tid] += add;
+        __syncthreads();
+    }
+}
+
+// Sort n pairs by key bits [0, 64).  tmp buffers must hold n entries; hist must hold
+// 256 * ceil(n / RS_TILE) entries.  The result ends in (keys, vals).
+inline void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, int64_t n,
+                             uint32_t* hist, hipStream_t st, int end_bit = 64)
+{
+    if (n <= 1) return;
+    const int ntiles = (int)ceil_div(n, RS_TILE);
+    uint64_t* kin = keys;
+    uint32_t* vin = vals;
+    uint64_t* kout = ktmp;
+    uint32_t* vout = vtmp;
+    int passes = 0;
+    for (int shift = 0; shift < end_bit; shift += 8, passes++) {
+        rs_hist_kernel<<<ntiles, RS_THREADS, 0, st>>>(kin, n, shift, hist, ntiles);
+        LVRHIP_LAUNCH_CHECK();
+        scan_exclusive_kernel<<<1, 1024, 0, st>>>(hist, (int64_t)RS_RADIX * ntiles, nullptr);
+        LVRHIP_LAUNCH_CHECK();
+        rs_scatter_kernel<<<ntiles, RS_THREADS, 0, st>>>(kin, vin, kout, vout, n, shift, hist, ntiles);
+        LVRHIP_LAUNCH_CHECK();
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (passes & 1) {
+        LVRHIP_CHECK(hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+        LVRHIP_CHECK(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    }
+}
+
+}  // namespace lvrhip

@@ -1,0 +1,335 @@
+// Consumers of the exact kNN sets produced by knn_tile_kernel.
+//
+//   SinkNormals: fused plane fit per query point (AdaptiveKSearchSurface.tcc:158-284):
+//                boundingBoxOK on the k-set, the selected method's plane normal, the flip
+//                towards the flip point, plus the sorted ki-prefix used by the
+//                interpolation (tcc:293-335).
+//   SinkSdf:     AdaptiveKSearchSurface::distance (tcc:414-457) + the invalid rule of
+//                PointsetGrid::calcDistanceValues (PointsetGrid.tcc:138-143).
+//   SinkList:    raw (d^2, index) keys for SearchTree::kSearch.
+//
+// Sinks see each selected neighbour once, on one of the P lanes of its query; finish()
+// runs on all lanes (it may shuffle within the query's lane group).
+#pragma once
+
+#include "common.hpp"
+#include "eig3.hpp"
+
+namespace lvrhip {
+
+template <int P>
+__device__ __forceinline__ float group_sum(float v)
+{
+#pragma unroll
+    for (int o = 1; o < P; o <<= 1) v += shfl_xor(v, o);
+    return v;
+}
+template <int P>
+__device__ __forceinline__ float group_min(float v)
+{
+#pragma unroll
+    for (int o = 1; o < P; o <<= 1) v = fminf(v, shfl_xor(v, o));
+    return v;
+}
+template <int P>
+__device__ __forceinline__ float group_max(float v)
+{
+#pragma unroll
+    for (int o = 1; o < P; o <<= 1) v = fmaxf(v, shfl_xor(v, o));
+    return v;
+}
+
+// Sort the n (key, payload) pairs of one query by key (keys are unique) with the P lanes of
+// the group: each lane ranks its elements against all others.
+template <int P>
+__device__ __forceinline__ void group_rank_sort(int s, int n, const uint64_t* keys, const uint32_t* payload,
+                                                uint32_t* out)
+{
+    for (int j = s; j < n; j += P) {
+        const uint64_t kj = keys[j];
+        int rank = 0;
+        for (int i = 0; i < n; i++) rank += keys[i] < kj ? 1 : 0;
+        out[rank] = payload[j];
+    }
+}
+
+// BaseVector::normalize (geometry/BaseVector.hpp:130-143) in fp32.
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z)
+{
+    if (!(x == 0.0f && y == 0.0f && z == 0.0f)) {
+        float l = sqrtf(x * x + y * y + z * z);
+        x /= l;
+        y /= l;
+        z /= l;
+    }
+}
+
+template <int P>
+struct SinkNormals {
+    // parameters
+    float* nx;
+    float* ny;
+    float* nz;
+    uint8_t* kround;       // per sorted position: adaptive round used (1..5)
+    uint32_t* retry;       // per query slot: 1 if the bbox test failed (NULL in the last round)
+    uint64_t* ki_keys;     // scratch [qpos * kiw + j]
+    uint32_t* ki_spos;     // scratch
+    uint32_t* ki_list;     // out [qpos * kiw + r], sorted by key
+    int kiw;
+    int round;             // 1..5, k = kn * 2^round
+    int kcount;            // |k-set| (= t0)
+    int method;
+    float fx, fy, fz;
+    // per-lane state
+    uint32_t qpos_, qi_;
+    float qx_, qy_, qz_;
+    float sx, sy, sz, sxx, sxy, sxz, syy, syz, szz;
+    float mnx, mny, mnz, mxx, mxy, mxz;
+
+    __device__ void begin(int g, int s, int64_t qi, uint32_t qpos, float qx, float qy, float qz)
+    {
+        qpos_ = qpos;
+        qi_ = (uint32_t)qi;
+        qx_ = qx;
+        qy_ = qy;
+        qz_ = qz;
+        sx = sy = sz = sxx = sxy = sxz = syy = syz = szz = 0.0f;
+        mnx = mny = mnz = INFINITY;
+        mxx = mxy = mxz = -INFINITY;
+    }
+
+    __device__ void accept(int tau, int g, int s, float x, float y, float z, uint64_t key, uint32_t spos,
+                           uint32_t* cnt)
+    {
+        if (tau == 0) {
+            const float rx = x - qx_, ry = y - qy_, rz = z - qz_;
+            sx += rx;
+            sy += ry;
+            sz += rz;
+            sxx += rx * rx;
+            sxy += rx * ry;
+            sxz += rx * rz;
+            syy += ry * ry;
+            syz += ry * rz;
+            szz += rz * rz;
+            mnx = fminf(mnx, x);
+            mny = fminf(mny, y);
+            mnz = fminf(mnz, z);
+            mxx = fmaxf(mxx, x);
+            mxy = fmaxf(mxy, y);
+            mxz = fmaxf(mxz, z);
+        } else {
+            const uint32_t slot = atomicAdd(&cnt[g], 1u);
+            if (slot < (uint32_t)kiw) {
+                ki_keys[(size_t)qpos_ * kiw + slot] = key;
+                ki_spos[(size_t)qpos_ * kiw + slot] = spos;
+            }
+        }
+    }
+
+    __device__ void finish(int g, int s, bool ok, uint32_t qpos, float qx, float qy, float qz, const uint32_t*,
+                           const uint32_t* cnt1)
+    {
+        sx = group_sum<P>(sx);
+        sy = group_sum<P>(sy);
+        sz = group_sum<P>(sz);
+        sxx = group_sum<P>(sxx);
+        sxy = group_sum<P>(sxy);
+        sxz = group_sum<P>(sxz);
+        syy = group_sum<P>(syy);
+        syz = group_sum<P>(syz);
+        szz = group_sum<P>(szz);
+        mnx = group_min<P>(mnx);
+        mny = group_min<P>(mny);
+        mnz = group_min<P>(mnz);
+        mxx = group_max<P>(mxx);
+        mxy = group_max<P>(mxy);
+        mxz = group_max<P>(mxz);
+        if (ok && s == 0) {
+            // boundingBoxOK, AdaptiveKSearchSurface.tcc:337-350
+            const float dx = mxx - mnx, dy = mxy - mny, dz = mxz - mnz;
+            const float e = 0.05f;
+            const bool bb_ok = dx >= e * dy && dx >= e * dz && dy >= e * dx && dy >= e * dz && dz >= e * dx &&
+                               dz >= e * dy;
+            if (!bb_ok && retry) {
+                retry[qi_] = 1u;
+            } else {
+                if (retry) retry[qi_] = 0u;
+                float n0, n1, n2;
+                plane_normal(n0, n1, n2);
+                // flip towards the flip point (tcc:268-274); Normal::operator- renormalises
+                const float ddx = fx - qx, ddy = fy - qy, ddz = fz - qz;
+                if (n0 * ddx + n1 * ddy + n2 * ddz < 0.0f) {
+                    n0 = -n0;
+                    n1 = -n1;
+                    n2 = -n2;
+                    normalize3(n0, n1, n2);
+                }
+                nx[qpos] = n0;
+                ny[qpos] = n1;
+                nz[qpos] = n2;
+                if (kround) kround[qpos] = (uint8_t)round;
+            }
+        }
+        if (kiw > 0 && ok) {
+            group_rank_sort<P>(s, min((int)cnt1[g], kiw), ki_keys + (size_t)qpos * kiw,
+                               ki_spos + (size_t)qpos * kiw, ki_list + (size_t)qpos * kiw);
+        }
+    }
+
+    __device__ void plane_normal(float& n0, float& n1, float& n2) const
+    {
+        if (method == 2) {
+            // calcPlaneIterative (tcc:522-607): second moments about the query point
+            const float xx = sxx, xy = sxy, xz = sxz, yy = syy, yz = syz, zz = szz;
+            const float det_x = yy * zz - yz * yz;
+            const float det_y = xx * zz - xz * xz;
+            const float det_z = xx * yy - xy * xy;
+            float dx, dy, dz;
+            if (det_x >= det_y && det_x >= det_z) {
+                dx = 1.0f;
+                dy = (xz * yz - xy * zz) / det_x;
+                dz = (xy * yz - xz * yy) / det_x;
+            } else if (det_y >= det_x && det_y >= det_z) {
+                dx = (yz * xz - xy * zz) / det_y;
+                dy = 1.0f;
+                dz = (xy * xz - yz * xx) / det_y;
+            } else {
+                dx = (yz * xy - xz * yy) / det_z;
+                dy = (xz * xy - yz * xx) / det_z;
+                dz = 1.0f;
+            }
+            const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+            n0 = dx * inv;
+            n1 = dy * inv;
+            n2 = dz * inv;
+        } else if (method == 0) {
+            // calcPlane (tcc:467-520): least squares y = c0 + c1 x + c2 z on the k-set,
+            // solved in fp64 from the query-centred moments
+            const double c = (double)kcount;
+            double A[3][3] = {{c, sx, sz}, {sx, sxx, sxz}, {sz, sxz, szz}};
+            double b[3] = {sy, sxy, syz};
+            solve3(A, b);
+            const double c1 = b[1], c2 = b[2];
+            const double c0 = (double)qy_ + b[0] - c1 * qx_ - c2 * qz_;
+            const float eps = 100.0f;
+            const float C0 = (float)c0, C1 = (float)c1, C2 = (float)c2;
+            const float z1 = C0 + C1 * (qx_ + eps) + C2 * qz_;
+            const float z2 = C0 + C1 * qx_ + C2 * (qz_ + eps);
+            const float d1x = (qx_ + eps) - qx_, d1y = z1 - qy_, d1z = qz_ - qz_;
+            const float d2x = qx_ - qx_, d2y = z2 - qy_, d2z = (qz_ + eps) - qz_;
+            n0 = d1y * d2z - d1z * d2y;
+            n1 = d1z * d2x - d1x * d2z;
+            n2 = d1x * d2y - d1y * d2x;
+            normalize3(n0, n1, n2);
+        } else {
+            // calcPlaneIPCAExact (tcc:609-673): covariance of {q} + k-set, smallest
+            // eigenvector.  Moments are taken about q, which the k-set contains.
+            const float cntf = (float)(kcount + 1);
+            const float mx = sx / cntf, my = sy / cntf, mz = sz / cntf;
+            float C[6];
+            C[0] = sxx / cntf - mx * mx;
+            C[1] = sxy / cntf - mx * my;
+            C[2] = sxz / cntf - mx * mz;
+            C[3] = syy / cntf - my * my;
+            C[4] = syz / cntf - my * mz;
+            C[5] = szz / cntf - mz * mz;
+            smallest_eigvec(C, n0, n1, n2);
+        }
+    }
+};
+
+template <int P>
+struct SinkSdf {
+    const float* px;
+    const float* py;
+    const float* pz;
+    const float* nx;
+    const float* ny;
+    const float* nz;
+    const uint32_t* qorig;  // query position -> caller's index
+    uint64_t* keys;         // scratch [qpos * kw + j]
+    uint32_t* spos;         // scratch
+    uint32_t* ord;          // scratch, sorted spos
+    int kw;
+    float voxel;
+    float* out_proj;
+    float* out_eucl;
+    uint8_t* out_invalid;
+    uint32_t npts;
+    uint32_t qpos_;
+
+    __device__ void begin(int g, int s, int64_t qi, uint32_t qpos, float, float, float) { qpos_ = qpos; }
+
+    __device__ void accept(int tau, int g, int s, float, float, float, uint64_t key, uint32_t sp, uint32_t* cnt)
+    {
+        if (tau != 0) return;
+        const uint32_t slot = atomicAdd(&cnt[g], 1u);
+        if (slot < (uint32_t)kw) {
+            keys[(size_t)qpos_ * kw + slot] = key;
+            spos[(size_t)qpos_ * kw + slot] = sp;
+        }
+    }
+
+    __device__ void finish(int g, int s, bool ok, uint32_t qpos, float qx, float qy, float qz, const uint32_t* cnt0,
+                           const uint32_t*)
+    {
+        const int cnt = min((int)cnt0[g], kw);
+        if (ok)
+            group_rank_sort<P>(s, cnt, keys + (size_t)qpos * kw, spos + (size_t)qpos * kw, ord + (size_t)qpos * kw);
+        __syncthreads();
+        if (ok && s == 0) {
+            // ordered fp32 sums, AdaptiveKSearchSurface.tcc:435-449
+            float ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+            const uint32_t* o = ord + (size_t)qpos * kw;
+            for (int j = 0; j < cnt; j++) {
+                const uint32_t sp = o[j];
+                if (sp >= npts) continue;
+                ax += px[sp];
+                ay += py[sp];
+                az += pz[sp];
+                bx += nx[sp];
+                by += ny[sp];
+                bz += nz[sp];
+            }
+            const float c = (float)cnt;
+            bx /= c;
+            by /= c;
+            bz /= c;
+            ax /= c;
+            ay /= c;
+            az /= c;
+            normalize3(bx, by, bz);
+            const float dx = qx - ax, dy = qy - ay, dz = qz - az;
+            const float proj = dx * bx + dy * by + dz * bz;
+            const float eucl = sqrtf(dx * dx + dy * dy + dz * dz);
+            const uint32_t oi = qorig ? qorig[qpos] : qpos;
+            if (out_proj) out_proj[oi] = proj;
+            if (out_eucl) out_eucl[oi] = eucl;
+            // PointsetGrid.tcc:138: the comparison is made in double
+            if (out_invalid) out_invalid[oi] = ((double)eucl > 1.75 * (double)voxel) ? 1 : 0;
+        }
+    }
+};
+
+template <int P>
+struct SinkList {
+    uint64_t* out;          // [qorig * kw + slot], unsorted
+    const uint32_t* qorig;
+    int kw;
+    uint32_t row_;
+
+    __device__ void begin(int g, int s, int64_t qi, uint32_t qpos, float, float, float)
+    {
+        row_ = qorig ? qorig[qpos] : qpos;
+    }
+    __device__ void accept(int tau, int g, int s, float, float, float, uint64_t key, uint32_t, uint32_t* cnt)
+    {
+        if (tau != 0) return;
+        const uint32_t slot = atomicAdd(&cnt[g], 1u);
+        if (slot < (uint32_t)kw) out[(size_t)row_ * kw + slot] = key;
+    }
+    __device__ void finish(int, int, bool, uint32_t, float, float, float, const uint32_t*, const uint32_t*) {}
+};
+
+}  // namespace lvrhip

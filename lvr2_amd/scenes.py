@@ -1,0 +1,75 @@
+"""Seeded synthetic point clouds (bindings of csrc/scenes.cpp) and the workload configs.
+
+Config names follow the workloads the path is measured on:
+  plain_small      300,000 points, plane + 3 boxes + sphere, sigma 0.5; kn10 ki10 kd5, voxel 10
+  room_4m          4,000,000 rays into a 20x15x4 m room with 40 objects, sigma 2 mm;
+                   kn50 ki20 kd10, flip = scanner origin, voxel 0.05 m
+  sphere_torus_1m  1,000,000 points on a noisy sphere + torus (sigma 1 % of R), kn30
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SCENES_PATH = os.path.join(HERE, "libscenes.so")
+_lib = None
+
+
+def _l():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SCENES_PATH):
+            raise RuntimeError(f"{SCENES_PATH} is not built; run `python -m lvr2_amd.build scenes`")
+        l = C.CDLL(SCENES_PATH)
+        l.scene_plane_boxes_sphere.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_void_p]
+        l.scene_room_scan.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]
+        l.scene_sphere_torus.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_void_p]
+        _lib = l
+    return _lib
+
+
+def plane_boxes_sphere(seed: int, n: int, sigma: float = 0.5) -> np.ndarray:
+    out = np.empty((n, 3), np.float32)
+    _l().scene_plane_boxes_sphere(seed, n, sigma, out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def room_scan(seed: int, n_az: int, n_el: int, n_obj: int = 40, sigma: float = 0.002) -> np.ndarray:
+    out = np.empty((n_az * n_el, 3), np.float32)
+    _l().scene_room_scan(seed, n_az, n_el, n_obj, sigma, out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def sphere_torus(seed: int, n: int, radius: float = 1.0, sigma_rel: float = 0.01) -> np.ndarray:
+    out = np.empty((n, 3), np.float32)
+    _l().scene_sphere_torus(seed, n, radius, sigma_rel, out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+@dataclass
+class Workload:
+    name: str
+    points: np.ndarray
+    kn: int
+    ki: int
+    kd: int
+    flip: tuple
+    voxel: float
+
+
+def plain_small(n: int = 300_000, seed: int = 1) -> Workload:
+    return Workload("plain_small", plane_boxes_sphere(seed, n, 0.5), 10, 10, 5, (0.0, 0.0, 0.0), 10.0)
+
+
+def room(n_az: int = 3200, n_el: int = 1250, seed: int = 2) -> Workload:
+    """Terrestrial-scan-like room; the default grid gives 4,000,000 rays."""
+    return Workload(f"room_{n_az * n_el}", room_scan(seed, n_az, n_el, 40, 0.002), 50, 20, 10, (0.0, 0.0, 0.0),
+                    0.05)
+
+
+def sphere_torus_1m(n: int = 1_000_000, seed: int = 5) -> Workload:
+    return Workload("sphere_torus", sphere_torus(seed, n, 1.0, 0.01), 30, 0, 10, (0.0, 0.0, 0.0), 0.05)

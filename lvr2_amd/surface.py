@@ -1,0 +1,186 @@
+"""HipSurface: the Python mirror of lvr2's CudaSurface / PointsetSurface call surface.
+
+Method names, argument meaning and defaults follow ``lvr2::CudaSurface``
+(include/lvr2/reconstruction/cuda/CudaSurface.hpp:70-158; defaults CudaSurface.cu:1007-1023)
+and the setters of ``lvr2::PointsetSurface`` (PointsetSurface.hpp:112-131).  Every call goes
+through the C ABI of liblvrhip.so.  Points may be given as a host array (N x 3 float32) or
+as a torch tensor on a ROCm device (the device-resident path, no host copy).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+
+METHODS = {"LS": 0, "PLANE": 0, "ITERATIVE": 2, "PCA": 3}
+
+
+def _as_host_points(a) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if a.ndim != 2 or a.shape[1] != 3:
+        raise ValueError("points must have shape (N, 3)")
+    return a
+
+
+def _is_device_tensor(a) -> bool:
+    return hasattr(a, "is_cuda") and bool(getattr(a, "is_cuda"))
+
+
+class HipSurface:
+    """Drop-in for ``CudaSurface(floatArr& points, size_t num_points, int device)``."""
+
+    def __init__(self, points, device: int = 0, stream=None):
+        self._h = C.c_void_p()
+        self._keep = None
+        if _is_device_tensor(points):
+            import torch
+
+            t = points.detach().to(torch.float32).contiguous()
+            if t.ndim != 2 or t.shape[1] != 3:
+                raise ValueError("points must have shape (N, 3)")
+            self._n = int(t.shape[0])
+            device = t.device.index if t.device.index is not None else device
+            st = stream if stream is not None else torch.cuda.current_stream(t.device).cuda_stream
+            check(lib().lvrhip_create_device(C.c_void_p(t.data_ptr()), self._n, device, C.c_void_p(st),
+                                             C.byref(self._h)))
+            torch.cuda.current_stream(t.device).synchronize()
+        else:
+            p = _as_host_points(points)
+            self._n = int(p.shape[0])
+            check(lib().lvrhip_create(p.ctypes.data_as(C.c_void_p), self._n, device, C.byref(self._h)))
+            if stream is not None:
+                self.setStream(stream)
+        self.device = device
+        self._kn, self._ki, self._kd = 10, 10, 5
+
+    # -- lifecycle ---------------------------------------------------------------------------
+    def freeGPU(self) -> None:
+        if self._h:
+            check(lib().lvrhip_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.freeGPU()
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return self._n
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    # -- parameters (CudaSurface.hpp:116-158) ---------------------------------------------------
+    def setKn(self, kn: int) -> None:
+        self._kn = int(kn)
+        self._push_k()
+
+    def setKi(self, ki: int) -> None:
+        self._ki = int(ki)
+        self._push_k()
+
+    def setKd(self, kd: int) -> None:
+        self._kd = int(kd)
+        self._push_k()
+
+    def _push_k(self) -> None:
+        check(lib().lvrhip_set_k(self._h, self._kn, self._ki, self._kd))
+
+    def setFlippoint(self, x: float, y: float, z: float) -> None:
+        check(lib().lvrhip_set_flip_point(self._h, float(x), float(y), float(z)))
+
+    def setMethod(self, method) -> None:
+        """"PCA" (calcPlaneIPCAExact), "ITERATIVE" (calcPlaneIterative), "LS" (calcPlane),
+        or the AdaptiveKSearchSurface method number 3 / 2 / 0."""
+        m = METHODS[method.upper()] if isinstance(method, str) else int(method)
+        check(lib().lvrhip_set_method(self._h, m))
+
+    def setStream(self, stream) -> None:
+        v = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        check(lib().lvrhip_set_stream(self._h, C.c_void_p(v or 0)))
+
+    # -- normals ---------------------------------------------------------------------------------
+    def calculateNormals(self) -> None:
+        check(lib().lvrhip_calculate_normals(self._h))
+
+    def interpolateNormals(self) -> None:
+        check(lib().lvrhip_interpolate_normals(self._h))
+
+    def setNormals(self, normals) -> None:
+        """Use given normals (input order) instead of calculateNormals."""
+        nrm = _as_host_points(normals)
+        if nrm.shape[0] != self._n:
+            raise ValueError("normals must have one row per point")
+        check(lib().lvrhip_set_normals(self._h, nrm.ctypes.data_as(C.c_void_p)))
+
+    def getNormals(self, out=None):
+        """Normals in input order.  ``out`` may be a host array or a device tensor (N, 3)."""
+        if out is not None and _is_device_tensor(out):
+            check(lib().lvrhip_get_normals_device(self._h, C.c_void_p(out.data_ptr())))
+            return out
+        res = np.empty((self._n, 3), np.float32) if out is None else out
+        check(lib().lvrhip_get_normals(self._h, res.ctypes.data_as(C.c_void_p)))
+        return res
+
+    def setProfiling(self, on: bool = True) -> None:
+        check(lib().lvrhip_set_profiling(self._h, int(bool(on))))
+
+    def timings(self, reset: bool = False) -> dict:
+        """{timer: (milliseconds, launches)} accumulated since the last reset."""
+        from ._lib import TIMERS
+
+        ms = np.zeros(len(TIMERS), np.float64)
+        cnt = np.zeros(len(TIMERS), np.int64)
+        check(lib().lvrhip_get_timings(self._h, ms.ctypes.data_as(C.c_void_p), cnt.ctypes.data_as(C.c_void_p),
+                                       int(bool(reset))))
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(TIMERS)}
+
+    def normalsStats(self) -> np.ndarray:
+        """Points searched in each adaptive round (k = kn*2, kn*4, ..., kn*32)."""
+        a = (C.c_int64 * 5)()
+        check(lib().lvrhip_normals_stats(self._h, a))
+        return np.array(list(a), np.int64)
+
+    # -- queries ----------------------------------------------------------------------------------
+    def kSearch(self, queries, k: int):
+        """SearchTree::kSearch for many queries: (idx int32 (m,k), d2 float32 (m,k))."""
+        q = _as_host_points(queries)
+        m = q.shape[0]
+        idx = np.empty((m, k), np.int32)
+        d2 = np.empty((m, k), np.float32)
+        check(lib().lvrhip_knn(self._h, q.ctypes.data_as(C.c_void_p), m, int(k),
+                               idx.ctypes.data_as(C.c_void_p), d2.ctypes.data_as(C.c_void_p)))
+        return idx, d2
+
+    def distances(self, query_points, voxel_size: float, out=None):
+        """CudaSurface::distances semantics: returns (proj, eucl, invalid) per query point.
+
+        ``query_points`` is (M, 3) float32 (host), or a device tensor; with a device tensor
+        ``out`` may be a tuple of device tensors (proj, eucl, invalid) to fill in place.
+        """
+        if _is_device_tensor(query_points):
+            import torch
+
+            qt = query_points.contiguous()
+            m = int(qt.shape[0])
+            if out is None:
+                out = (torch.empty(m, dtype=torch.float32, device=qt.device),
+                       torch.empty(m, dtype=torch.float32, device=qt.device),
+                       torch.empty(m, dtype=torch.uint8, device=qt.device))
+            check(lib().lvrhip_distances_device(self._h, C.c_void_p(qt.data_ptr()), m, float(voxel_size),
+                                                C.c_void_p(out[0].data_ptr()), C.c_void_p(out[1].data_ptr()),
+                                                C.c_void_p(out[2].data_ptr())))
+            return out
+        q = _as_host_points(query_points)
+        m = q.shape[0]
+        proj = np.empty(m, np.float32)
+        eucl = np.empty(m, np.float32)
+        inv = np.empty(m, np.uint8)
+        check(lib().lvrhip_distances(self._h, q.ctypes.data_as(C.c_void_p), m, float(voxel_size),
+                                     proj.ctypes.data_as(C.c_void_p), eucl.ctypes.data_as(C.c_void_p),
+                                     inv.ctypes.data_as(C.c_void_p)))
+        return proj, eucl, inv

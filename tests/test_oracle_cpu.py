@@ -1,0 +1,202 @@
+"""Pins the CPU oracle (oracle/lvr2_oracle.cpp) before it is trusted as the checker.
+
+lvr2 ships no tests or golden vectors for this path, so the oracle is pinned against
+independent numpy computations (tests/golden/*.npz), brute force, closed forms and
+invariants, chosen so that a dropped term, a wrong sign or a transposed operand fails.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, angles, axis_angles
+
+
+def test_knn_matches_numpy_golden(oracle_mod):
+    g = np.load(os.path.join(GOLDEN, "knn_golden.npz"))
+    for name in ("gauss_dups", "lattice", "tiny"):
+        pts, q = g[f"{name}_points"], g[f"{name}_queries"]
+        s = oracle_mod.Surface(pts)
+        idx, d2 = s.knn(q, 16)
+        np.testing.assert_array_equal(idx, g[f"{name}_idx"], err_msg=name)
+        np.testing.assert_array_equal(d2, g[f"{name}_d2"], err_msg=name)
+        bi, bd = oracle_mod.knn_brute(pts, q, 16)
+        np.testing.assert_array_equal(bi, g[f"{name}_idx"], err_msg=name)
+        np.testing.assert_array_equal(bd, g[f"{name}_d2"], err_msg=name)
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (2, 5), (25, 25), (300, 7), (5000, 64)])
+def test_kdtree_equals_brute_force(oracle_mod, n, k):
+    rng = np.random.default_rng(n * 31 + k)
+    p = rng.normal(size=(n, 3)).astype(np.float32)
+    p = np.round(p * 4) / 4  # many exact ties
+    q = np.concatenate([p[: min(n, 50)], rng.normal(size=(20, 3)).astype(np.float32)])
+    s = oracle_mod.Surface(p)
+    i1, d1 = s.knn(q, k)
+    i2, d2 = oracle_mod.knn_brute(p, q, k)
+    np.testing.assert_array_equal(i1, i2)
+    np.testing.assert_array_equal(d1, d2)
+
+
+def test_reference_metric_agrees_off_ties(oracle_mod):
+    # The reference's fp64 distances and heap tie order give the same sets as the contract
+    # wherever the k-th and (k+1)-th distances are not near-ties.
+    rng = np.random.default_rng(5)
+    p = rng.normal(size=(4000, 3)).astype(np.float32)
+    q = p[:300]
+    ic, dc = oracle_mod.Surface(p).knn(q, 21)
+    ir, _ = oracle_mod.Surface(p, oracle_mod.REFERENCE).knn(q, 20)
+    gap = (dc[:, 20] - dc[:, 19]) / np.maximum(dc[:, 20], 1e-30)
+    clear = gap > 1e-5
+    for r in np.where(clear)[0]:
+        assert set(ic[r, :20]) == set(ir[r])
+
+
+def test_eig3_against_numpy(oracle_mod):
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        b = rng.normal(size=(3, 3))
+        a = b @ b.T
+        a[2] *= rng.uniform(1e-3, 1)
+        a[:, 2] = a[2]
+        w, v = oracle_mod.eig3(a, fp64=True)
+        W, V = np.linalg.eigh(a)
+        np.testing.assert_allclose(np.sort(w), W, rtol=1e-10, atol=1e-12)
+        c = int(np.argmin(w))
+        assert axis_angles(v[:, c][None], V[:, 0][None])[0] < 1e-9
+        # fp32 variant on the same matrix
+        wf, vf = oracle_mod.eig3(a.astype(np.float32))
+        cf = int(np.argmin(wf))
+        gap = (W[1] - W[0]) / W[2]
+        assert axis_angles(vf[:, cf][None], V[:, 0][None])[0] < 1e-5 / max(gap, 1e-3)
+
+
+@pytest.mark.parametrize("method,key", [(3, "n3"), (2, "n2"), (0, "n0")])
+def test_plane_fits_match_numpy_golden(oracle_mod, method, key):
+    g = np.load(os.path.join(GOLDEN, "planes_golden.npz"))
+    for c in range(g["points"].shape[0]):
+        pts = g["points"][c]
+        flip = g["flips"][c]
+        s = oracle_mod.Surface(pts)
+        # kn = 10: the first search (k = 20) already returns the whole 20-point cloud
+        n32, n64, k = s.normals(10, method, flip, subset=[0], fp64=True)
+        # method 0 builds its normal from two tangents 100 units long: fp64 noise ~1e-9
+        assert angles(n64, g[key][c][None])[0] < (1e-7 if method == 0 else 1e-9), (c, n64, g[key][c])
+        tol = 1e-3 if method == 3 else 1e-5  # fp32 IPCA accumulates rounding over the recursion
+        assert angles(n32, g[key][c][None])[0] < tol, (c, n32, g[key][c])
+
+
+def test_noise_free_plane_and_sphere_closed_forms(oracle_mod):
+    rng = np.random.default_rng(3)
+    # plane z = 0.3 x - 0.2 y + 4, tilted so no bbox extent is zero
+    xy = rng.uniform(-10, 10, size=(3000, 2))
+    z = 0.3 * xy[:, 0] - 0.2 * xy[:, 1] + 4
+    pts = np.column_stack([xy, z]).astype(np.float32)
+    nt = np.array([-0.3, 0.2, 1.0]) / np.linalg.norm([-0.3, 0.2, 1.0])
+    s = oracle_mod.Surface(pts)
+    for method in (3, 2, 0):
+        n32, n64, k = s.normals(5, method, (0, 0, 100), subset=np.arange(0, 3000, 37), fp64=True)
+        assert np.max(angles(n64, np.tile(nt, (n64.shape[0], 1)))) < 1e-6
+    # sphere of radius 5, flip at the centre -> normals point inwards
+    d = rng.normal(size=(20000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    sp = (5 * d).astype(np.float32)
+    s = oracle_mod.Surface(sp)
+    n32, n64, k = s.normals(10, 3, (0, 0, 0), subset=np.arange(0, 20000, 97), fp64=True)
+    assert np.max(angles(n64, -d[::97])) < 0.05  # chord sag of a 20-point cap
+
+
+def test_adaptive_k_rounds(oracle_mod):
+    rng = np.random.default_rng(4)
+    # an axis-aligned noise-free plane never passes boundingBoxOK -> last round, k = 32 kn
+    xy = rng.uniform(0, 10, size=(5000, 2))
+    flat = np.column_stack([xy, np.zeros(5000)]).astype(np.float32)
+    _, _, k = oracle_mod.Surface(flat).normals(4, 3, (0, 0, 1), subset=np.arange(0, 5000, 50))
+    assert np.all(k == 4 * 32)
+    # an isotropic blob passes at the first search, k = 2 kn
+    blob = rng.normal(size=(5000, 3)).astype(np.float32)
+    _, _, k = oracle_mod.Surface(blob).normals(4, 3, (0, 0, 0), subset=np.arange(0, 5000, 50))
+    assert np.all(k == 8)
+
+
+def test_flip_orientation(oracle_mod):
+    rng = np.random.default_rng(8)
+    xy = rng.uniform(-1, 1, size=(2000, 2))
+    pts = np.column_stack([xy, 0.01 * rng.normal(size=2000)]).astype(np.float32)
+    s = oracle_mod.Surface(pts)
+    up, _, _ = s.normals(10, 3, (0, 0, 50))
+    down, _, _ = s.normals(10, 3, (0, 0, -50))
+    assert np.all(up[:, 2] > 0) and np.all(down[:, 2] < 0)
+    np.testing.assert_allclose(up, -down, atol=1e-6)
+
+
+def test_interpolation_hand_case(oracle_mod):
+    # four points on a line; ki = 2 -> each point sums its own normal, itself and its
+    # nearest other point (AdaptiveKSearchSurface.tcc:317-324)
+    pts = np.array([[0, 0, 0], [1, 0, 0], [3, 0, 0], [7, 0, 0]], np.float32)
+    nrm = np.array([[0, 0, 1], [0, 1, 0], [1, 0, 0], [0, 0, -1]], np.float32)
+    out = oracle_mod.Surface(pts).interpolate(nrm, 2)
+
+    def nz(v):
+        v = np.asarray(v, np.float64)
+        return v / np.linalg.norm(v)
+
+    exp = np.array([nz(2 * nrm[0] + nrm[1]), nz(2 * nrm[1] + nrm[0]), nz(2 * nrm[2] + nrm[1]),
+                    nz(2 * nrm[3] + nrm[2])])
+    np.testing.assert_allclose(out, exp, atol=1e-7)
+
+
+def test_distance_closed_form(oracle_mod):
+    # plane z = 0 sampled on a lattice, normals +z: proj of (x, y, h) is exactly h
+    g = np.stack(np.meshgrid(np.arange(-20, 21), np.arange(-20, 21), indexing="ij"), -1).reshape(-1, 2)
+    pts = np.column_stack([g, np.zeros(len(g))]).astype(np.float32)
+    nrm = np.tile(np.array([0, 0, 1], np.float32), (len(g), 1))
+    s = oracle_mod.Surface(pts)
+    q = np.array([[0.0, 0.0, 0.75], [0.0, 0.0, -2.5], [3.0, 4.0, 1.0]], np.float32)
+    proj, eucl = s.distance(nrm, 1, q)
+    np.testing.assert_array_equal(proj, q[:, 2])
+    np.testing.assert_allclose(eucl, np.abs(q[:, 2]))
+    # kd = 5 around a lattice node: mean of the node and its 4 neighbours is the node
+    proj, eucl = s.distance(nrm, 5, q)
+    np.testing.assert_allclose(proj, q[:, 2], atol=1e-6)
+
+
+def test_grid_matches_brute_force(oracle_mod):
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(0, 4, size=(200, 3)).astype(np.float32)
+    bb = np.concatenate([pts.min(0), pts.max(0)])
+    voxel = 0.7
+    v, cells, cidx = oracle_mod.grid(pts, bb, voxel, extrude=True)
+
+    # brute force: cells of the points whose centre lies in the box, extruded by one
+    def center(i):
+        return (np.asarray(i, np.float32) + np.float32(0.5)) * np.float32(voxel)
+
+    def inside(c):
+        return np.all(bb[:3] <= c) and np.all(bb[3:] >= c)
+
+    base = {tuple(np.floor(p / np.float32(voxel)).astype(int)) for p in pts}
+    base = {c for c in base if inside(center(c))}
+    ext = set(base)
+    for c in base:
+        for d in np.ndindex(3, 3, 3):
+            n = tuple(np.array(c) + np.array(d) - 1)
+            if inside(center(n)):
+                ext.add(n)
+    assert {tuple(c) for c in cidx} == ext
+    # vertex set = all cell corners, each once
+    corners = {(c[0] + a, c[1] + b, c[2] + e) for c in ext for a in (0, 1) for b in (0, 1) for e in (0, 1)}
+    assert len(v) == len(corners)
+    lat = np.rint(v / np.float32(voxel)).astype(int)
+    assert {tuple(x) for x in lat} == corners
+    # the 8 vertex ids of a cell sit at its corners in box_creation_table order
+    bct = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+    np.testing.assert_array_equal(lat[cells] - cidx[:, None, :], np.broadcast_to(bct, (len(cells), 8, 3)))
+
+
+def test_calc_distance_values_rule(oracle_mod):
+    eucl = np.array([0.0, 1.74, 1.75, 1.7500001, 5.0], np.float32)
+    cells = np.array([[0, 1, 2, 0, 1, 2, 0, 1], [0, 1, 2, 3, 0, 1, 2, 3]], np.int32)
+    inv, keep = oracle_mod.calc_distance_values(eucl, 1.0, cells)
+    np.testing.assert_array_equal(inv, [0, 0, 0, 1, 1])
+    np.testing.assert_array_equal(keep, [1, 0])
