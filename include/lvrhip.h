@@ -143,6 +143,19 @@ enum {
     LVRHIP_T_COUNT = 6
 };
 int lvrhip_set_profiling(lvrhip_t h, int on);
+
+/* Kernel work counters collected while profiling is on (no lvr2 counterpart):
+ * out[g * 16 + c] for group g (0 first normal round, 1 later rounds, 2 SDF, 3 other kNN) and
+ * counter c (0 tiles, 1 radius attempts, 2 LDS-resident attempts, 3 candidate points
+ * scanned in the histogram pass, 4 refinement passes, 5 chunk loads, 6 queries resolved,
+ * 7 traversal node pops; 8..15 shader-clock cycles summed over waves in: traversal, chunk
+ * staging, per-query leaf lists, histogram scan, selection, collection scan, finish, total). */
+int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[64], int reset);
+
+/* Launch shape of the kNN tile kernels (no lvr2 counterpart): lanes per query (2, 4 or 8;
+ * queries per wavefront = 64 / lanes), consecutive tiles per wavefront, and the radius
+ * margin carried from one tile to the next.  Values <= 0 keep the current setting. */
+int lvrhip_set_tuning(lvrhip_t h, int lanes_per_query, int tiles_per_wave, float r2_margin);
 /* ms[i], launches[i] for i < LVRHIP_T_COUNT; either may be NULL.  reset != 0 clears them. */
 int lvrhip_get_timings(lvrhip_t h, double* ms, int64_t* launches, int reset);
 

@@ -34,7 +34,13 @@ _SIGS = {
     "lvrhip_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "lvrhip_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "lvrhip_get_timings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "lvrhip_get_kernel_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "lvrhip_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float]),
 }
+
+STAT_GROUPS = ("normals_first", "normals_retry", "sdf", "other")
+STAT_COUNTERS = ("tiles", "attempts", "resident", "candidates", "refine", "chunks", "resolved", "pops",
+                 "cy_trav", "cy_load", "cy_glist", "cy_hist", "cy_select", "cy_collect", "cy_finish", "cy_total")
 
 TIMERS = ("normals_first", "normals_retry", "interpolate", "sdf_kernel", "sdf_sort", "build")
 
@@ -51,11 +57,22 @@ class LvrHipError(RuntimeError):
         self.code = code
 
 
+def _bind_process_hip_runtime() -> None:
+    """One HIP runtime per process.  torch ships its own libamdhip64.so.7; when torch is
+    importable it is imported first so that liblvrhip.so's libamdhip64.so.7 dependency
+    resolves to that already-loaded runtime instead of loading a second one."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run `python -m lvr2_amd.build`")
+        _bind_process_hip_runtime()
         l = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(l, name)

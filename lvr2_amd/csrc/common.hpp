@@ -79,6 +79,13 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
+__device__ __forceinline__ int wave_sum_int(int v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
+    return v;
+}
+
 // Squared distance of the kNN contract: (dx*dx + dy*dy) + dz*dz in fp32 with every
 // operation rounded (the library is built with -ffp-contract=off).
 __device__ __forceinline__ float d2_contract(float qx, float qy, float qz, float px, float py, float pz)

@@ -5,20 +5,24 @@
 // it must agree with (KDTree.tcc:99-126, 186-248).
 //
 // A wave owns Q = 64/P consecutive queries, P lanes per query.  For a radius R it
-//   1. walks the LBVH once for the whole tile (wave-parallel, LDS stack), collecting the
-//      leaves whose box is within R of the tile's box;
-//   2. stages the candidate points in LDS (chunks of CAP points; when the candidates fit in
-//      one chunk they stay resident for all passes);
-//   3. per query, counts candidates with d^2 <= R^2 into 32 bins linear in d^2 (one LDS
-//      column per lane).  If fewer than t candidates are in range the query keeps R
-//      pending and the tile retries with 2R; otherwise the bin holding the t-th smallest
-//      key is found, refined by radix passes over the 64-bit key (d^2 bits, point index)
-//      until it holds at most BUF keys, and a collection pass hands every key below that
+//   1. walks the LBVH once for the whole tile (wave-parallel, LDS stack), collecting in LDS
+//      the leaves whose box is within R of the tile's box;
+//   2. stages the points of those leaves in LDS (chunks of CAP points; when they fit in one
+//      chunk they stay resident for all passes);
+//   3. gives every query its own list of the chunk's leaves whose box is within that
+//      query's current bound, and scans only those, the P lanes of the query splitting
+//      each leaf's points;
+//   4. per query, counts candidates with d^2 <= R^2 into 32 bins linear in d^2.  If fewer
+//      than t candidates are in range the query stays pending and the tile retries with
+//      2R; otherwise the bin holding the t-th smallest key is found, refined by radix
+//      passes over the 64-bit key (d^2 bits, point index) until it holds at most BUF keys,
+//      and a collection pass (bounded by that bin's upper edge) hands every key below the
 //      bin to the sink, buffers the bin, and ranks the buffer exactly.
-// Every point within R of a query is a candidate (the box distance is a monotone lower
-// bound of the fp32 d^2), so a query with >= t in-range candidates gets the exact t
-// smallest keys.  Two targets are selected at once (e.g. k for the plane fit and ki for
-// the interpolation list).
+// Every point within R of a query is a candidate (box distances are monotone lower bounds of
+// the fp32 d^2), so a query with >= t in-range candidates gets the exact t smallest keys.
+// Two targets are selected at once (e.g. k for the plane fit and ki for the interpolation
+// list).  If a tile's leaf list outgrows LDS, each pass re-walks the tree and streams the
+// leaves chunk by chunk instead.
 #pragma once
 
 #include "common.hpp"
@@ -26,10 +30,12 @@
 
 namespace lvrhip {
 
-constexpr int NB = 32;            // histogram bins
-constexpr int CAP = 512;          // candidate slots per chunk
+constexpr int NB = 32;         // histogram bins
+constexpr int CAP = 512;       // candidate slots per chunk
 constexpr int CHUNK_LEAVES = CAP / LEAF;
+constexpr int TL_CAP = 512;    // leaves kept per tile
 constexpr int STACK_CAP = 512;
+constexpr int STACK_WIDE = 256;
 constexpr int MAX_ATTEMPTS = 80;
 
 struct BvhView {
@@ -56,17 +62,65 @@ struct TileParams {
     int t0, t1;  // targets (t1 <= t0; t1 = 0 for none), already clamped to n
     int tiles_per_wave;
     float r2_init;
-    float r2_margin;  // next tile's radius^2 = margin * largest k-th d^2 of this tile
-    int* err;  // set to a nonzero code on capacity overflow
+    float r2_margin;            // next tile's radius^2 = margin * largest k-th d^2 of this tile
+    int* err;                   // set to a nonzero code on capacity overflow
+    unsigned long long* stats;  // optional counters, see KS_* below
+};
+
+enum {
+    KS_TILES = 0,   // tiles processed
+    KS_ATTEMPTS,    // radius attempts (>= tiles)
+    KS_RESIDENT,    // attempts whose candidates fit one LDS chunk
+    KS_CANDIDATES,  // (query, leaf) pairs scanned by the histogram pass, in points
+    KS_REFINE,      // refinement passes (wave-level)
+    KS_CHUNKS,      // chunk loads over all passes
+    KS_RESOLVED,    // queries resolved
+    KS_POPS,        // stack entries expanded by the traversal
+    // shader-clock cycles per phase, summed over waves
+    KS_CY_TRAV,     // tile traversal
+    KS_CY_LOAD,     // chunk staging
+    KS_CY_GLIST,    // per-query leaf lists
+    KS_CY_HIST,     // histogram pass (scan only)
+    KS_CY_SELECT,   // bin search + refinement passes
+    KS_CY_COLLECT,  // collection pass (scan only)
+    KS_CY_FINISH,   // boundary ranking + sink finish
+    KS_CY_TOTAL,    // whole wave
+    KS_COUNT
+};
+
+__device__ __forceinline__ unsigned long long cycles() { return (unsigned long long)clock64(); }
+
+// Wave-local counters, flushed with one atomic per counter at the end of the wave.
+struct WaveStats {
+    unsigned long long c[KS_COUNT];
+    bool timing;
+    __device__ void clear()
+    {
+#pragma unroll
+        for (int i = 0; i < KS_COUNT; i++) c[i] = 0;
+    }
+    __device__ __forceinline__ unsigned long long tick() const { return timing ? cycles() : 0ull; }
+    __device__ __forceinline__ void tock(int i, unsigned long long t0)
+    {
+        if (timing) c[i] += cycles() - t0;
+    }
+    __device__ void flush(unsigned long long* dst)
+    {
+        if (!dst || lane_id() != 0) return;
+#pragma unroll
+        for (int i = 0; i < KS_COUNT; i++)
+            if (c[i]) atomicAdd(&dst[i], c[i]);
+    }
 };
 
 template <int Q, int BUF>
 struct TileSmem {
     float4 cand[CAP];  // x, y, z, original index (as bits)
-    uint32_t cspos[CAP];
-    uint32_t hist[NB][kWave];
+    float4 lbox[CHUNK_LEAVES][2];  // boxes of the chunk's leaves (lo, hi)
+    uint32_t hist[Q][NB];
     int stack[STACK_CAP];
-    int chunk_leaf[CHUNK_LEAVES];
+    int tleaf[TL_CAP];                // the tile's leaves (or the current chunk's, when streaming)
+    uint8_t glist[Q][CHUNK_LEAVES];   // per query: chunk leaf slots within its bound
     uint64_t bkey[2][Q][BUF];
     uint32_t bspos[2][Q][BUF];
     uint32_t bcnt[2][Q];
@@ -89,22 +143,27 @@ struct TileBox {
     float lx, ly, lz, hx, hy, hz;
 };
 
-// Fills sm.chunk_leaf with up to CHUNK_LEAVES leaves whose box is within sqrt(r2) of the
-// tile box, continuing the traversal held on the LDS stack.  Returns the number of
-// leaves (wave-uniform).  sp is the stack size (wave-uniform), -1 on overflow.
+// Appends to sm.tleaf[0..cap) the leaves whose box is within sqrt(r2) of the tile box,
+// continuing the traversal held on the LDS stack (sp entries).  Stops when the stack is
+// empty or the list is full; returns the list length (wave-uniform).  sp = -1 on overflow.
 template <class Sm>
-__device__ int fill_chunk(Sm& sm, const BvhView& bv, const TileBox& tb, float r2, int& sp)
+__device__ int fill_leaves(Sm& sm, const BvhView& bv, const TileBox& tb, float r2, int& sp, int cap, WaveStats& ws)
 {
     const int lane = lane_id();
     int nl = 0;
     while (sp > 0) {
-        const int room = CHUNK_LEAVES - nl;
+        const int room = cap - nl;
         if (room == 0) break;
-        int w = min(min(kWave, sp), STACK_CAP - sp);
+        // Wide pops (up to 64 entries, up to 128 pushes) while STACK_WIDE slots are free;
+        // below that, depth-first pops of one entry, whose growth is bounded by the tree
+        // depth (< STACK_WIDE - 2 * kWave).
+        const int free_slots = STACK_CAP - sp;
+        const int w = free_slots >= STACK_WIDE ? min(kWave, sp) : (free_slots >= 2 ? 1 : 0);
         if (w <= 0) {
             sp = -1;
             return nl;
         }
+        ws.c[KS_POPS] += (unsigned)w;
         int ref = lane < w ? sm.stack[sp - 1 - lane] : 0x7fffffff;
         sp -= w;
         __syncthreads();
@@ -131,8 +190,8 @@ __device__ int fill_chunk(Sm& sm, const BvhView& bv, const TileBox& tb, float r2
         const int r0 = mbcnt(bl0), r1 = __popcll(bl0) + mbcnt(bl1);
         bool push0 = (o0 != 0x7fffffff) && (!l0 || r0 >= take);
         bool push1 = (o1 != 0x7fffffff) && (!l1 || r1 >= take);
-        if (l0 && r0 < take) sm.chunk_leaf[nl + r0] = -1 - o0;
-        if (l1 && r1 < take) sm.chunk_leaf[nl + r1] = -1 - o1;
+        if (l0 && r0 < take) sm.tleaf[nl + r0] = -1 - o0;
+        if (l1 && r1 < take) sm.tleaf[nl + r1] = -1 - o1;
         const uint64_t bp0 = ballot(push0), bp1 = ballot(push1);
         if (push0) sm.stack[sp + mbcnt(bp0)] = o0;
         if (push1) sm.stack[sp + __popcll(bp0) + mbcnt(bp1)] = o1;
@@ -143,47 +202,163 @@ __device__ int fill_chunk(Sm& sm, const BvhView& bv, const TileBox& tb, float r2
     return nl;
 }
 
-// Stage the points of the chunk's leaves in LDS; slot = leaf_slot * LEAF + offset.
+// Stage the points of leaves sm.tleaf[base, base + nl) in LDS; slot = j * LEAF + offset.
 // Slots past the end of the cloud get NaN coordinates (never in range).
 template <class Sm>
-__device__ void load_chunk(Sm& sm, const BvhView& bv, int nl)
+__device__ void load_chunk(Sm& sm, const BvhView& bv, int base, int nl)
 {
+    constexpr int LPI = kWave / LEAF;             // leaves per iteration
+    constexpr int IT = CHUNK_LEAVES / LPI;         // iterations for a full chunk
     const int lane = lane_id();
-    for (int j = 0; j < nl; j += 2) {
-        const int slot_leaf = j + (lane >> 5);
-        if (slot_leaf < nl) {
-            const int leaf = sm.chunk_leaf[slot_leaf];
-            const uint32_t spos = (uint32_t)leaf * LEAF + (lane & 31);
-            const int slot = slot_leaf * LEAF + (lane & 31);
-            if (spos < bv.n) {
-                sm.cand[slot] = make_float4(bv.px[spos], bv.py[spos], bv.pz[spos], __uint_as_float(bv.pidx[spos]));
-                sm.cspos[slot] = spos;
-            } else {
-                sm.cand[slot] = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.0f);
-                sm.cspos[slot] = 0;
-            }
+    float4 v[IT];
+    // all loads first, then the LDS stores: IT global loads in flight per lane
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int jl = it * LPI + lane / LEAF;
+        v[it] = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.0f);
+        if (jl < nl) {
+            const uint32_t spos = (uint32_t)sm.tleaf[base + jl] * LEAF + (lane % LEAF);
+            if (spos < bv.n) v[it] = make_float4(bv.px[spos], bv.py[spos], bv.pz[spos], __uint_as_float(bv.pidx[spos]));
         }
     }
+    float4 bl = make_float4(0, 0, 0, 0), bh = make_float4(0, 0, 0, 0);
+    if (lane < nl) {
+        const int leaf = sm.tleaf[base + lane];
+        bl = bv.leaf_lo[leaf];
+        bh = bv.leaf_hi[leaf];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int jl = it * LPI + lane / LEAF;
+        if (jl < nl) sm.cand[jl * LEAF + (lane % LEAF)] = v[it];
+    }
+    if (lane < nl) {
+        sm.lbox[lane][0] = bl;
+        sm.lbox[lane][1] = bh;
+    }
     __syncthreads();
+}
+
+// sorted position of candidate slot c of the chunk whose leaves start at sm.tleaf[base]
+template <class Sm>
+__device__ __forceinline__ uint32_t slot_spos(const Sm& sm, int base, int c)
+{
+    return (uint32_t)sm.tleaf[base + c / LEAF] * LEAF + (uint32_t)(c % LEAF);
+}
+
+// Per query group g: the chunk's leaves (sm.tleaf[base, base+nl)) whose box lies within
+// sqrt(rq) of the query point, as chunk slots in sm.glist[g]; returns their count
+// (replicated over the group's lanes).  rq < 0 disables the query.
+template <int P, class Sm>
+__device__ int build_glist(Sm& sm, const BvhView& bv, int base, int nl, int g, int s, float qx, float qy, float qz,
+                           float rq)
+{
+    constexpr uint64_t GM = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
+    int len = 0;
+    for (int j0 = 0; j0 < nl; j0 += P) {
+        const int j = j0 + s;
+        bool hit = false;
+        if (j < nl && rq >= 0.0f) {
+            const int leaf = sm.tleaf[base + j];
+            const float4 lo = sm.lbox[j][0], hi = sm.lbox[j][1];
+            hit = box_d2(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, qx, qy, qz, qx, qy, qz) <= rq;
+        }
+        const uint32_t gm = (uint32_t)((ballot(hit) >> (g * P)) & GM);
+        if (hit) sm.glist[g][len + __popc(gm & ((1u << s) - 1u))] = (uint8_t)j;
+        len += __popc(gm);
+    }
+    __syncthreads();
+    return len;
+}
+
+// The candidates of one radius attempt: either the tile's leaf list (ntl >= 0; resident when
+// it fits one chunk) or, after a list overflow, a fresh traversal per pass.
+struct Cands {
+    int ntl;        // leaves in sm.tleaf, or -1 when streaming
+    bool resident;  // all candidates already staged in sm.cand
+};
+
+// Calls f(nl, base) once per staged chunk of nl leaves starting at sm.tleaf[base].
+template <class Sm, class F>
+__device__ bool for_each_chunk(Sm& sm, const BvhView& bv, const TileBox& tb, float R2, int root, Cands& cs,
+                               WaveStats& ws, F&& f)
+{
+    if (cs.ntl >= 0) {
+        if (cs.resident) {
+            f(cs.ntl, 0);
+            __syncthreads();
+            return true;
+        }
+        for (int base = 0; base < cs.ntl; base += CHUNK_LEAVES) {
+            const int nl = min(CHUNK_LEAVES, cs.ntl - base);
+            const auto t0 = ws.tick();
+            load_chunk(sm, bv, base, nl);
+            ws.tock(KS_CY_LOAD, t0);
+            ws.c[KS_CHUNKS] += 1;
+            f(nl, base);
+            __syncthreads();
+        }
+        if (cs.ntl <= CHUNK_LEAVES) cs.resident = true;
+        return true;
+    }
+    int sp = 1;
+    if (lane_id() == 0) sm.stack[0] = root;
+    __syncthreads();
+    for (;;) {
+        auto t0 = ws.tick();
+        const int nl = fill_leaves(sm, bv, tb, R2, sp, CHUNK_LEAVES, ws);
+        ws.tock(KS_CY_TRAV, t0);
+        if (sp < 0) return false;
+        t0 = ws.tick();
+        load_chunk(sm, bv, 0, nl);
+        ws.tock(KS_CY_LOAD, t0);
+        ws.c[KS_CHUNKS] += 1;
+        f(nl, 0);
+        __syncthreads();
+        if (sp == 0) return true;
+    }
+}
+
+// Scan the points of query group g's listed leaves: fn(p, slot) for each, the P lanes of
+// the group taking every P-th point of a leaf (LEAF / P independent LDS reads per lane).
+template <int P, class Sm, class Fn>
+__device__ __forceinline__ void scan_glist(const Sm& sm, int g, int s, int len, Fn&& fn)
+{
+    constexpr int U = LEAF / P;  // points per lane per leaf
+    constexpr int L = U >= 4 ? 1 : 4 / U;  // leaves per step: at least 4 LDS reads in flight
+    static_assert(LEAF % P == 0, "lanes per query must divide the leaf size");
+    for (int j = 0; j < len; j += L) {
+        int slot0[L];
+        float4 p[L][U];
+#pragma unroll
+        for (int l = 0; l < L; l++) {
+            slot0[l] = (j + l < len) ? (int)sm.glist[g][j + l] * LEAF + s : -1;
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                p[l][u] = slot0[l] >= 0 ? sm.cand[slot0[l] + u * P]
+                                        : make_float4(__builtin_nanf(""), 0.0f, 0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int l = 0; l < L; l++)
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (slot0[l] >= 0) fn(p[l][u], slot0[l] + u * P);
+    }
 }
 
 // ---- histogram helpers ----------------------------------------------------------------------
 
 // Locate the bin holding the t-th smallest counted key of query group g (P lanes).
 // Returns total count; if t <= total: bin, below (count in lower bins), cnt (in that bin).
-template <int P>
-__device__ __forceinline__ int find_bin(const uint32_t (*hist)[kWave], int g, int s, int t, int& bin, int& below,
+template <int P, int Q>
+__device__ __forceinline__ int find_bin(const uint32_t (*hist)[NB], int g, int s, int t, int& bin, int& below,
                                         int& cnt)
 {
     constexpr int BPL = NB / P;
     static_assert(NB % P == 0, "bins per lane");
     int sum = 0;
 #pragma unroll
-    for (int j = 0; j < BPL; j++) {
-        const int b = s * BPL + j;
-#pragma unroll
-        for (int c = 0; c < P; c++) sum += (int)hist[b][g * P + c];
-    }
+    for (int j = 0; j < BPL; j++) sum += (int)hist[g][s * BPL + j];
     int incl = sum;
 #pragma unroll
     for (int o = 1; o < P; o <<= 1) {
@@ -199,9 +374,7 @@ __device__ __forceinline__ int find_bin(const uint32_t (*hist)[kWave], int g, in
 #pragma unroll
         for (int j = 0; j < BPL; j++) {
             const int b = s * BPL + j;
-            int v = 0;
-#pragma unroll
-            for (int c = 0; c < P; c++) v += (int)hist[b][g * P + c];
+            const int v = (int)hist[g][b];
             if (run < t && t <= run + v) {
                 b_ = b;
                 below_ = run;
@@ -252,12 +425,16 @@ template <int P, class Sink>
 __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView qv, TileParams prm, Sink sink)
 {
     constexpr int Q = kWave / P;
-    constexpr int BUF = (Q >= 64) ? 8 : (Q >= 32 ? 12 : 16);
+    constexpr int BUF = 12;
     __shared__ TileSmem<Q, BUF> sm;
     const int lane = lane_id();
     const int g = lane / P, s = lane % P;
-    const int root = bv.nleaf > 1 ? 0 : -1;  // internal node 0, or the single leaf
+    const int root = bv.nleaf > 1 ? 0 : -1;  // internal node 0, or the single leaf (ref -1)
     float r2 = fmaxf(prm.r2_init, 1e-30f);
+    WaveStats ws;
+    ws.clear();
+    ws.timing = prm.stats != nullptr;
+    const auto t_wave = ws.tick();
 
     for (int tt = 0; tt < prm.tiles_per_wave; tt++) {
         const int64_t tile = (int64_t)blockIdx.x * prm.tiles_per_wave + tt;
@@ -270,6 +447,7 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
         const float qy = valid ? qv.qy[qpos] : 0.0f;
         const float qz = valid ? qv.qz[qpos] : 0.0f;
         bool active = valid && prm.t0 > 0;
+        ws.c[KS_TILES] += 1;
         sink.begin(g, s, qi, qpos, qx, qy, qz);
         if (s == 0) {
             sm.scnt[0][g] = 0u;
@@ -289,44 +467,53 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
             tb.hz = uniformf(wave_max(active ? qz : -INFINITY));
             const float R2 = r2;
             const float inv_w = (float)NB / R2;
+            ws.c[KS_ATTEMPTS] += 1;
+
+            // --- candidate leaves of the tile -------------------------------------------------
+            Cands cs;
+            {
+                int sp = 1;
+                if (lane == 0) sm.stack[0] = root;
+                __syncthreads();
+                const auto t0 = ws.tick();
+                const int nl = fill_leaves(sm, bv, tb, R2, sp, TL_CAP, ws);
+                ws.tock(KS_CY_TRAV, t0);
+                if (sp < 0) {
+                    if (lane == 0) atomicMax(prm.err, 5);
+                    return;
+                }
+                cs.ntl = sp == 0 ? nl : -1;
+                cs.resident = false;
+            }
 
             // --- pass 1: value histogram over [0, R2] ---------------------------------------
-            int sp = 0, nl = 0;
-            bool resident = false;
-#pragma unroll
-            for (int b = 0; b < NB; b++) sm.hist[b][lane] = 0u;
-            if (lane == 0) sm.stack[0] = root >= 0 ? 0 : -1;  // leaf 0 -> ref -1
-            sp = 1;
+            for (int b = s; b < NB; b += P) sm.hist[g][b] = 0u;
             __syncthreads();
-            bool first_chunk = true;
-            for (;;) {
-                nl = fill_chunk(sm, bv, tb, R2, sp);
-                if (sp < 0) break;
-                if (first_chunk && sp == 0) resident = true;
-                first_chunk = false;
-                load_chunk(sm, bv, nl);
-                if (active) {
-                    const int C = nl * LEAF;
-                    for (int c = s; c < C; c += P) {
-                        const float4 p = sm.cand[c];
-                        const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
-                        if (d2 <= R2) atomicAdd(&sm.hist[vbin(d2, inv_w)][lane], 1u);
-                    }
-                }
-                __syncthreads();
-                if (sp == 0) break;
-            }
-            if (sp < 0) {
+            bool fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
+                auto t0 = ws.tick();
+                const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, active ? R2 : -1.0f);
+                ws.tock(KS_CY_GLIST, t0);
+                t0 = ws.tick();
+                scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
+                    const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
+                    if (d2 <= R2) atomicAdd(&sm.hist[g][vbin(d2, inv_w)], 1u);
+                });
+                ws.tock(KS_CY_HIST, t0);
+                if (prm.stats) ws.c[KS_CANDIDATES] += (unsigned)(LEAF * wave_sum_int(s == 0 ? len : 0));
+            });
+            if (!fine) {
                 if (lane == 0) atomicMax(prm.err, 5);
                 return;
             }
+            if (cs.resident) ws.c[KS_RESIDENT] += 1;
 
             // --- targets -----------------------------------------------------------------------
+            const auto t_sel = ws.tick();
             Target tg[2];
             bool ok = false;
             {
                 int b, below, cnt;
-                const int total = find_bin<P>(sm.hist, g, s, prm.t0, b, below, cnt);
+                const int total = find_bin<P, Q>(sm.hist, g, s, prm.t0, b, below, cnt);
                 ok = active && total >= prm.t0;
 #pragma unroll
                 for (int tau = 0; tau < 2; tau++) {
@@ -338,7 +525,7 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
                     tg[tau].klo = ~0ull;
                     tg[tau].khi = ~0ull;
                     if (tau == 1) {
-                        if (t > 0) find_bin<P>(sm.hist, g, s, t, b, below, cnt);
+                        if (t > 0) find_bin<P, Q>(sm.hist, g, s, t, b, below, cnt);
                     }
                     if (ok && t > 0) {
                         const float lo = b == 0 ? 0.0f : vbin_floor(b, inv_w);
@@ -362,40 +549,29 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
                 for (int guard = 0; guard < 16; guard++) {
                     const bool need = ok && tg[tau].refine;
                     if (ballot(need) == 0) break;
+                    ws.c[KS_REFINE] += 1;
                     const uint64_t span = tg[tau].khi - tg[tau].klo;
                     const int nbits = span ? 64 - __clzll((long long)span) : 0;
                     const int shift = nbits > 5 ? nbits - 5 : 0;
-#pragma unroll
-                    for (int b = 0; b < NB; b++) sm.hist[b][lane] = 0u;
+                    const float rq = need ? bitsf((uint32_t)(tg[tau].khi >> 32)) : -1.0f;
+                    for (int b = s; b < NB; b += P) sm.hist[g][b] = 0u;
                     __syncthreads();
-                    auto count = [&](int C) {
-                        if (!need) return;
-                        for (int c = s; c < C; c += P) {
-                            const float4 p = sm.cand[c];
+                    fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
+                        const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rq);
+                        scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
                             const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
-                            if (!(d2 <= R2)) continue;
+                            if (!(d2 <= R2)) return;
                             const uint64_t key = make_key(d2, __float_as_uint(p.w));
                             if (key >= tg[tau].klo && key <= tg[tau].khi)
-                                atomicAdd(&sm.hist[(int)((key - tg[tau].klo) >> shift)][lane], 1u);
-                        }
-                    };
-                    if (resident) {
-                        count(nl * LEAF);
-                        __syncthreads();
-                    } else {
-                        if (lane == 0) sm.stack[0] = root >= 0 ? 0 : -1;
-                        sp = 1;
-                        __syncthreads();
-                        for (;;) {
-                            nl = fill_chunk(sm, bv, tb, R2, sp);
-                            load_chunk(sm, bv, nl);
-                            count(nl * LEAF);
-                            __syncthreads();
-                            if (sp <= 0) break;
-                        }
+                                atomicAdd(&sm.hist[g][(int)((key - tg[tau].klo) >> shift)], 1u);
+                        });
+                    });
+                    if (!fine) {
+                        if (lane == 0) atomicMax(prm.err, 5);
+                        return;
                     }
                     int b, below, cnt;
-                    find_bin<P>(sm.hist, g, s, tg[tau].t - tg[tau].below, b, below, cnt);
+                    find_bin<P, Q>(sm.hist, g, s, tg[tau].t - tg[tau].below, b, below, cnt);
                     if (need) {
                         const uint64_t nlo = tg[tau].klo + ((uint64_t)b << shift);
                         uint64_t nhi = nlo + ((shift >= 64) ? ~0ull : ((1ull << shift) - 1ull));
@@ -414,56 +590,52 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
                 tg[tau].dhi = bitsf((uint32_t)(tg[tau].khi >> 32));
             }
 
-            // --- collection pass -------------------------------------------------------------------
+            ws.tock(KS_CY_SELECT, t_sel);
+
+            // --- collection pass, bounded by the last bin's upper edge ---------------------------------
             if (s == 0) {
                 sm.bcnt[0][g] = 0;
                 sm.bcnt[1][g] = 0;
             }
             __syncthreads();
-            auto collect = [&](int C) {
-                if (!ok) return;
-                for (int c = s; c < C; c += P) {
-                    const float4 p = sm.cand[c];
+            const float rc = ok ? fmaxf(tg[0].dhi, prm.t1 > 0 ? tg[1].dhi : 0.0f) : -1.0f;
+            fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
+                auto t0 = ws.tick();
+                const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rc);
+                ws.tock(KS_CY_GLIST, t0);
+                t0 = ws.tick();
+                scan_glist<P>(sm, g, s, len, [&](const float4& p, int c) {
                     const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
-                    if (!(d2 <= R2)) continue;
+                    if (!(d2 <= R2)) return;
 #pragma unroll
                     for (int tau = 0; tau < 2; tau++) {
                         if (tg[tau].t <= 0) continue;
                         if (d2 < tg[tau].dlo) {
-                            sink.accept(tau, g, s, p.x, p.y, p.z, make_key(d2, __float_as_uint(p.w)), sm.cspos[c],
-                                        sm.scnt[tau]);
+                            sink.accept(tau, g, s, p.x, p.y, p.z, make_key(d2, __float_as_uint(p.w)),
+                                        slot_spos(sm, base, c), sm.scnt[tau]);
                         } else if (d2 <= tg[tau].dhi) {
                             const uint64_t key = make_key(d2, __float_as_uint(p.w));
                             if (key < tg[tau].klo) {
-                                sink.accept(tau, g, s, p.x, p.y, p.z, key, sm.cspos[c], sm.scnt[tau]);
+                                sink.accept(tau, g, s, p.x, p.y, p.z, key, slot_spos(sm, base, c), sm.scnt[tau]);
                             } else if (key <= tg[tau].khi) {
                                 const uint32_t slot = atomicAdd(&sm.bcnt[tau][g], 1u);
                                 if (slot < (uint32_t)BUF) {
                                     sm.bkey[tau][g][slot] = key;
-                                    sm.bspos[tau][g][slot] = sm.cspos[c];
+                                    sm.bspos[tau][g][slot] = slot_spos(sm, base, c);
                                 }
                             }
                         }
                     }
-                }
-            };
-            if (resident) {
-                collect(nl * LEAF);
-                __syncthreads();
-            } else {
-                if (lane == 0) sm.stack[0] = root >= 0 ? 0 : -1;
-                sp = 1;
-                __syncthreads();
-                for (;;) {
-                    nl = fill_chunk(sm, bv, tb, R2, sp);
-                    load_chunk(sm, bv, nl);
-                    collect(nl * LEAF);
-                    __syncthreads();
-                    if (sp <= 0) break;
-                }
+                });
+                ws.tock(KS_CY_COLLECT, t0);
+            });
+            if (!fine) {
+                if (lane == 0) atomicMax(prm.err, 5);
+                return;
             }
 
             // --- exact ranking inside the boundary bins --------------------------------------------
+            const auto t_fin = ws.tick();
             float kth = 0.0f;
             if (ok) {
 #pragma unroll
@@ -485,8 +657,10 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
             }
             __syncthreads();
             sink.finish(g, s, ok, qpos, qx, qy, qz, sm.scnt[0], sm.scnt[1]);
+            ws.tock(KS_CY_FINISH, t_fin);
             // largest kth distance of the tile: next tile's radius estimate
             kth_max = fmaxf(kth_max, uniformf(wave_max(ok ? kth : 0.0f)));
+            ws.c[KS_RESOLVED] += (unsigned)__popcll(ballot(ok && s == 0));
             active = active && !ok;
             if (ballot(active) != 0) {
                 float nr2 = R2 * 4.0f;
@@ -503,6 +677,8 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
         if (kth_max > 0.0f) r2 = kth_max * prm.r2_margin;
         r2 = fmaxf(r2, 1e-30f);
     }
+    ws.tock(KS_CY_TOTAL, t_wave);
+    ws.flush(prm.stats);
 }
 
 }  // namespace lvrhip

@@ -14,7 +14,7 @@
 
 namespace lvrhip {
 
-constexpr int LEAF = 32;
+constexpr int LEAF = 16;
 
 struct alignas(16) BvhNode {
     float4 a;  // lo0.x lo0.y lo0.z hi0.x
@@ -123,20 +123,20 @@ __global__ __launch_bounds__(256) void gather_sorted_kernel(const float* __restr
     if (rank) rank[j] = (uint32_t)i;
 }
 
-// Leaf boxes: one lane per point, 32 lanes per leaf; also the leaf codes.
+// Leaf boxes: one lane per point, LEAF lanes per leaf; also the leaf codes.
 __global__ __launch_bounds__(256) void leaf_boxes_kernel(const float* __restrict__ px, const float* __restrict__ py,
                                                          const float* __restrict__ pz, int64_t n,
                                                          const uint64_t* __restrict__ codes, int nleaf,
                                                          float4* __restrict__ leaf_lo, float4* __restrict__ leaf_hi,
                                                          uint64_t* __restrict__ leaf_code)
 {
-    static_assert(LEAF == 32, "leaf reduction assumes 32 lanes per leaf");
+    static_assert(LEAF >= 2 && LEAF <= kWave && (LEAF & (LEAF - 1)) == 0, "leaf = power of two lanes");
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = i < n;
     float x = valid ? px[i] : INFINITY, y = valid ? py[i] : INFINITY, z = valid ? pz[i] : INFINITY;
     float X = valid ? px[i] : -INFINITY, Y = valid ? py[i] : -INFINITY, Z = valid ? pz[i] : -INFINITY;
 #pragma unroll
-    for (int m = 16; m >= 1; m >>= 1) {
+    for (int m = LEAF / 2; m >= 1; m >>= 1) {
         x = fminf(x, shfl_xor(x, m));
         y = fminf(y, shfl_xor(y, m));
         z = fminf(z, shfl_xor(z, m));

@@ -97,38 +97,107 @@ __global__ void gather_queries_kernel(const float* __restrict__ q, int64_t m, co
     qz[i] = q[3 * (int64_t)j + 2];
 }
 
-// next round's query list: entries whose retry flag is set, in order
-__global__ void compact_kernel(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ scan, int64_t nq,
-                               const uint32_t* __restrict__ qsel, uint32_t* __restrict__ out)
+// Ordered compaction of 0/1 flags (the next adaptive round's query list):
+// per-4096 block counts, one scan of the block counts, then a block-local stable scatter.
+constexpr int CF_THREADS = 256;
+constexpr int CF_ITEMS = 16;
+constexpr int CF_TILE = CF_THREADS * CF_ITEMS;
+
+__global__ __launch_bounds__(CF_THREADS) void flag_block_sums_kernel(const uint32_t* __restrict__ flags, int64_t n,
+                                                                     uint32_t* __restrict__ bsum)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    if (flags[i]) out[scan[i]] = qsel ? qsel[i] : (uint32_t)i;
+    __shared__ uint32_t ws[CF_THREADS / kWave];
+    const int64_t base = (int64_t)blockIdx.x * CF_TILE;
+    uint32_t c = 0;
+    for (int r = 0; r < CF_ITEMS; r++) {
+        const int64_t i = base + (int64_t)r * CF_THREADS + threadIdx.x;
+        if (i < n && flags[i]) c++;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) c += shfl_xor(c, m);
+    if (lane_id() == 0) ws[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < CF_THREADS / kWave; w++) t += ws[w];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(CF_THREADS) void compact_scatter_kernel(const uint32_t* __restrict__ flags, int64_t n,
+                                                                     const uint32_t* __restrict__ boff,
+                                                                     const uint32_t* __restrict__ qsel,
+                                                                     uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t wc[CF_THREADS / kWave];
+    const int64_t base = (int64_t)blockIdx.x * CF_TILE;
+    const int wave = threadIdx.x / kWave;
+    uint32_t run = boff[blockIdx.x];
+    for (int r = 0; r < CF_ITEMS; r++) {
+        const int64_t i = base + (int64_t)r * CF_THREADS + threadIdx.x;
+        const bool f = i < n && flags[i] != 0;
+        const uint64_t m = ballot(f);
+        if (lane_id() == 0) wc[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (int w = 0; w < CF_THREADS / kWave; w++) {
+            if (w < wave) pre += wc[w];
+            tot += wc[w];
+        }
+        if (f) out[run + pre + (uint32_t)mbcnt(m)] = qsel ? qsel[i] : (uint32_t)i;
+        run += tot;
+        __syncthreads();
+    }
 }
 
 // interpolateSurfaceNormals (AdaptiveKSearchSurface.tcc:312-327): the point's own normal
-// plus its ki nearest (ascending, self included), normalised, into a fresh array.
-__global__ __launch_bounds__(256) void interpolate_kernel(const float* __restrict__ nx, const float* __restrict__ ny,
-                                                          const float* __restrict__ nz,
-                                                          const uint32_t* __restrict__ ki_list, int kiw, int64_t n,
-                                                          float* __restrict__ ox, float* __restrict__ oy,
-                                                          float* __restrict__ oz)
+// plus its ki nearest (self included), summed in ascending (d^2, index) order like the
+// reference's kSearch result, normalised, into a fresh array.  G lanes per point: the
+// kNN pass stored the ki list unordered, so each lane recomputes one neighbour's key, the
+// list is ranked in LDS (and written back sorted), then one lane sums in order.
+constexpr int INTERP_THREADS = 256;
+
+template <int G>
+__global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
+    const float* __restrict__ nx, const float* __restrict__ ny, const float* __restrict__ nz,
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const uint32_t* __restrict__ pidx, uint32_t* __restrict__ ki_list, int kiw, int64_t n, float* __restrict__ ox,
+    float* __restrict__ oy, float* __restrict__ oz)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float x = nx[i], y = ny[i], z = nz[i];
-    const uint32_t* l = ki_list + (size_t)i * kiw;
-    for (int j = 0; j < kiw; j++) {
-        const uint32_t s = l[j];
-        if (s >= n) continue;
-        x += nx[s];
-        y += ny[s];
-        z += nz[s];
+    __shared__ uint64_t skey[INTERP_THREADS];
+    __shared__ uint32_t sord[INTERP_THREADS];
+    const int grp = threadIdx.x / G, j = threadIdx.x % G, base = grp * G;
+    const int64_t i = (int64_t)blockIdx.x * (INTERP_THREADS / G) + grp;
+    const bool valid = i < n;
+    uint32_t s = 0;
+    uint64_t key = ~0ull;
+    if (valid && j < kiw) {
+        s = ki_list[(size_t)i * kiw + j];
+        if (s < n) key = make_key(d2_contract(px[i], py[i], pz[i], px[s], py[s], pz[s]), pidx[s]);
+        skey[base + j] = key;
     }
-    normalize3(x, y, z);
-    ox[i] = x;
-    oy[i] = y;
-    oz[i] = z;
+    __syncthreads();
+    if (valid && j < kiw) {
+        int rank = 0;
+        for (int t = 0; t < kiw; t++) rank += skey[base + t] < key ? 1 : 0;
+        sord[base + rank] = s;
+        ki_list[(size_t)i * kiw + rank] = s;
+    }
+    __syncthreads();
+    if (valid && j == 0) {
+        float x = nx[i], y = ny[i], z = nz[i];
+        for (int r = 0; r < kiw; r++) {
+            const uint32_t t = sord[base + r];
+            if (t >= n) continue;
+            x += nx[t];
+            y += ny[t];
+            z += nz[t];
+        }
+        normalize3(x, y, z);
+        ox[i] = x;
+        oy[i] = y;
+        oz[i] = z;
+    }
 }
 
 __global__ void gather_normals_kernel(const float* __restrict__ in, const uint32_t* __restrict__ pidx, int64_t n,
@@ -175,30 +244,21 @@ __global__ __launch_bounds__(kWave) void sort_rows_kernel(const uint64_t* __rest
     }
 }
 
-// ki list without normals (used when ki > 2*kn, so the first normal round cannot supply it)
+// ki list without normals (used when ki > 2*kn, so the first normal round cannot supply it);
+// unordered, like the one written by SinkNormals.
 template <int P>
 struct SinkKiOnly {
-    uint64_t* keys;
-    uint32_t* spos;
     uint32_t* out;
     int kw;
     uint32_t qpos_;
     __device__ void begin(int, int, int64_t, uint32_t qpos, float, float, float) { qpos_ = qpos; }
-    __device__ void accept(int tau, int g, int, float, float, float, uint64_t key, uint32_t sp, uint32_t* cnt)
+    __device__ void accept(int tau, int g, int, float, float, float, uint64_t, uint32_t sp, uint32_t* cnt)
     {
         if (tau != 0) return;
         const uint32_t slot = atomicAdd(&cnt[g], 1u);
-        if (slot < (uint32_t)kw) {
-            keys[(size_t)qpos_ * kw + slot] = key;
-            spos[(size_t)qpos_ * kw + slot] = sp;
-        }
+        if (slot < (uint32_t)kw) out[(size_t)qpos_ * kw + slot] = sp;
     }
-    __device__ void finish(int g, int s, bool ok, uint32_t qpos, float, float, float, const uint32_t* cnt0,
-                           const uint32_t*)
-    {
-        if (ok)
-            group_rank_sort<P>(s, min((int)cnt0[g], kw), keys + (size_t)qpos * kw, spos + (size_t)qpos * kw, out + (size_t)qpos * kw);
-    }
+    __device__ void finish(int, int, bool, uint32_t, float, float, float, const uint32_t*, const uint32_t*) {}
 };
 
 // hipEvent pairs around kernels on the handle's stream, accumulated per timer.
@@ -280,9 +340,9 @@ struct lvrhip_surface {
     int64_t n = 0;
     int kn = 10, ki = 10, kd = 5, method = LVRHIP_METHOD_PCA;
     float flip[3] = {1e6f, 1e6f, 1e6f};
-    int lanes_per_query = 4;
+    int lanes_per_query = 8;
     int tiles_per_wave = 8;
-    float r2_margin = 2.0f;
+    float r2_margin = 1.3f;
 
     DBuf<float> px, py, pz;
     DBuf<uint32_t> pidx, rank;
@@ -299,9 +359,22 @@ struct lvrhip_surface {
     int kiw = 0;
     bool ki_valid = false;
     DBuf<uint8_t> kround;
+    DBuf<float> mom;  // per point: moments of its k-set about the point (9 floats)
+    DBuf<uint32_t> s_bsum;
     int64_t n_round[5] = {0, 0, 0, 0, 0};
     DBuf<int> err;
     Profiler prof;
+    DBuf<unsigned long long> kstats;  // 4 groups x KS_COUNT
+
+    unsigned long long* kstats_ptr(int group)
+    {
+        if (!prof.on) return nullptr;
+        if (!kstats.p) {
+            kstats.reserve(4 * KS_COUNT);
+            LVRHIP_CHECK(hipMemsetAsync(kstats.p, 0, 4 * KS_COUNT * sizeof(unsigned long long), st));
+        }
+        return kstats.p + group * KS_COUNT;
+    }
 
     // scratch
     DBuf<uint64_t> s_k0, s_k1;
@@ -432,6 +505,19 @@ struct lvrhip_surface {
         LVRHIP_CHECK(hipStreamSynchronize(st));
     }
 
+    // out = the entries i < nq with flags[i] != 0 (qsel[i] or i), in order; *count_dev = count
+    void compact_flags(const uint32_t* flags, int64_t nq, const uint32_t* qsel, uint32_t* out, uint32_t* count_dev)
+    {
+        const int64_t nb = ceil_div(nq, CF_TILE);
+        s_bsum.reserve(nb + 1);
+        flag_block_sums_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, s_bsum.p);
+        LVRHIP_LAUNCH_CHECK();
+        scan_exclusive_kernel<<<1, 1024, 0, st>>>(s_bsum.p, nb, count_dev);
+        LVRHIP_LAUNCH_CHECK();
+        compact_scatter_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, s_bsum.p, qsel, out);
+        LVRHIP_LAUNCH_CHECK();
+    }
+
     void calculate_normals()
     {
         const int64_t N = n;
@@ -446,12 +532,9 @@ struct lvrhip_surface {
         ny.reserve(N);
         nz.reserve(N);
         kround.reserve(N);
+        mom.reserve((size_t)N * 9);
         kiw = (ki > 0 && ki <= 2 * kn) ? (int)std::min<int64_t>(ki, N) : 0;
-        if (kiw > 0) {
-            ki_list.reserve((size_t)N * kiw);
-            s_k0.reserve((size_t)N * kiw);
-            s_spos.reserve((size_t)N * kiw);
-        }
+        if (kiw > 0) ki_list.reserve((size_t)N * kiw);
         s_flags.reserve(N);
         s_scan.reserve(N + 1);
         s_list0.reserve(N);
@@ -469,42 +552,32 @@ struct lvrhip_surface {
             prm.r2_init = r2_init(t0);
             prm.r2_margin = r2_margin;
             prm.err = err.p;
+            prm.stats = kstats_ptr(r == 1 ? 0 : 1);
             QueryView qv{px.p, py.p, pz.p, qsel, (uint32_t)nq};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
                 SinkNormals<P> s{};
-                s.nx = nx.p;
-                s.ny = ny.p;
-                s.nz = nz.p;
-                s.kround = kround.p;
+                s.mom = mom.p;
                 s.retry = r < 5 ? s_flags.p : nullptr;
-                s.ki_keys = s_k0.p;
-                s.ki_spos = s_spos.p;
                 s.ki_list = ki_list.p;
                 s.kiw = prm.t1;
-                s.round = r;
-                s.kcount = t0;
-                s.method = method;
-                s.fx = flip[0];
-                s.fy = flip[1];
-                s.fz = flip[2];
                 return s;
             };
             const int tok = prof.start(r == 1 ? LVRHIP_T_NORMALS_FIRST : LVRHIP_T_NORMALS_RETRY, st);
             tile<SinkNormals>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
                               mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+            plane_fit_kernel<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(
+                qsel, nq, r < 5 ? s_flags.p : nullptr, mom.p, px.p, py.p, pz.p, method, t0, flip[0], flip[1], flip[2],
+                nx.p, ny.p, nz.p, kround.p, r);
+            LVRHIP_LAUNCH_CHECK();
             n_round[r - 1] = nq;
             if (r == 5) {
                 prof.stop(tok, st);
                 break;
             }
             // failing queries -> next list
-            LVRHIP_CHECK(hipMemcpyAsync(s_scan.p, s_flags.p, nq * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-            scan_exclusive_kernel<<<1, 1024, 0, st>>>(s_scan.p, nq, s_scan.p + N);
-            LVRHIP_LAUNCH_CHECK();
             uint32_t* out = lists[r & 1];
-            compact_kernel<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(s_flags.p, s_scan.p, nq, qsel, out);
-            LVRHIP_LAUNCH_CHECK();
+            compact_flags(s_flags.p, nq, qsel, out, s_scan.p + N);
             prof.stop(tok, st);
             uint32_t cnt = 0;
             LVRHIP_CHECK(hipMemcpyAsync(&cnt, s_scan.p + N, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -526,8 +599,6 @@ struct lvrhip_surface {
         if (ki_valid && kiw == kw) return;
         kiw = kw;
         ki_list.reserve((size_t)N * kw);
-        s_k0.reserve((size_t)N * kw);
-        s_spos.reserve((size_t)N * kw);
         TileParams prm{};
         prm.t0 = kw;
         prm.t1 = 0;
@@ -535,12 +606,11 @@ struct lvrhip_surface {
         prm.r2_init = r2_init(kw);
         prm.r2_margin = r2_margin;
         prm.err = err.p;
+        prm.stats = kstats_ptr(3);
         QueryView qv{px.p, py.p, pz.p, nullptr, (uint32_t)N};
         auto mk = [&](auto tag) {
             constexpr int P = decltype(tag)::value;
             SinkKiOnly<P> s{};
-            s.keys = s_k0.p;
-            s.spos = s_spos.p;
             s.out = ki_list.p;
             s.kw = kw;
             return s;
@@ -555,14 +625,23 @@ struct lvrhip_surface {
     {
         if (!has_normals) throw Error(LVRHIP_ESTATE, "interpolate_normals called before calculate_normals");
         if (ki <= 0 || n == 0) return;
+        if (std::min<int64_t>(ki, n) > 64) throw Error(LVRHIP_EINVAL, "ki above 64 is not supported");
         ensure_ki_list();
         const int64_t N = n;
         tx.reserve(N);
         ty.reserve(N);
         tz.reserve(N);
         const int tok = prof.start(LVRHIP_T_INTERPOLATE, st);
-        interpolate_kernel<<<(unsigned)ceil_div(N, 256), 256, 0, st>>>(nx.p, ny.p, nz.p, ki_list.p, kiw, N, tx.p,
-                                                                      ty.p, tz.p);
+        if (kiw <= 16) {
+            interpolate_kernel<16><<<(unsigned)ceil_div(N, INTERP_THREADS / 16), INTERP_THREADS, 0, st>>>(
+                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, N, tx.p, ty.p, tz.p);
+        } else if (kiw <= 32) {
+            interpolate_kernel<32><<<(unsigned)ceil_div(N, INTERP_THREADS / 32), INTERP_THREADS, 0, st>>>(
+                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, N, tx.p, ty.p, tz.p);
+        } else {
+            interpolate_kernel<64><<<(unsigned)ceil_div(N, INTERP_THREADS / 64), INTERP_THREADS, 0, st>>>(
+                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, N, tx.p, ty.p, tz.p);
+        }
         LVRHIP_LAUNCH_CHECK();
         prof.stop(tok, st);
         if (prof.on) prof.collect();
@@ -612,6 +691,7 @@ struct lvrhip_surface {
         prm.r2_init = r2_init(t0);
         prm.r2_margin = r2_margin;
         prm.err = err.p;
+        prm.stats = kstats_ptr(2);
         QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m};
         auto mk = [&](auto tag) {
             constexpr int P = decltype(tag)::value;
@@ -657,6 +737,7 @@ struct lvrhip_surface {
             prm.r2_init = r2_init(t0);
             prm.r2_margin = r2_margin;
             prm.err = err.p;
+            prm.stats = kstats_ptr(3);
             QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
@@ -956,6 +1037,44 @@ int lvrhip_set_profiling(lvrhip_t h, int on)
         DeviceGuard dg(h->device);
         h->prof.collect();
         h->prof.on = on != 0;
+    });
+}
+
+int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[64], int reset)
+{
+    return guarded([&] {
+        REQUIRE(h && out, "NULL argument");
+        DeviceGuard dg(h->device);
+        unsigned long long v[4 * KS_COUNT] = {};
+        if (h->kstats.p) {
+            LVRHIP_CHECK(hipMemcpyAsync(v, h->kstats.p, sizeof(v), hipMemcpyDeviceToHost, h->st));
+            LVRHIP_CHECK(hipStreamSynchronize(h->st));
+            if (reset) {
+                LVRHIP_CHECK(hipMemsetAsync(h->kstats.p, 0, sizeof(v), h->st));
+                LVRHIP_CHECK(hipStreamSynchronize(h->st));
+            }
+        }
+        for (int i = 0; i < 4 * KS_COUNT; i++) out[i] = (int64_t)v[i];
+    });
+}
+
+int lvrhip_set_tuning(lvrhip_t h, int lanes_per_query, int tiles_per_wave, float r2_margin)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        if (lanes_per_query > 0) {
+            REQUIRE(lanes_per_query == 2 || lanes_per_query == 4 || lanes_per_query == 8,
+                    "lanes per query must be 2, 4 or 8");
+            h->lanes_per_query = lanes_per_query;
+        }
+        if (tiles_per_wave > 0) {
+            REQUIRE(tiles_per_wave <= 4096, "tiles per wave out of range");
+            h->tiles_per_wave = tiles_per_wave;
+        }
+        if (r2_margin > 0.0f) {
+            REQUIRE(r2_margin >= 1.0f && r2_margin <= 1e6f, "radius margin out of range");
+            h->r2_margin = r2_margin;
+        }
     });
 }
 

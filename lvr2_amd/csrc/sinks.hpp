@@ -64,22 +64,82 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z)
     }
 }
 
+// Plane normal from the moments of {q} + k-set about q:
+// m = (sx sy sz sxx sxy sxz syy syz szz), kcount = |k-set|.
+__device__ inline void plane_from_moments(int method, int kcount, float qx, float qy, float qz, const float* m,
+                                          float& n0, float& n1, float& n2)
+{
+    const float sx = m[0], sy = m[1], sz = m[2], sxx = m[3], sxy = m[4], sxz = m[5], syy = m[6], syz = m[7],
+                szz = m[8];
+    if (method == 2) {
+        // calcPlaneIterative (tcc:522-607): second moments about the query point
+        const float xx = sxx, xy = sxy, xz = sxz, yy = syy, yz = syz, zz = szz;
+        const float det_x = yy * zz - yz * yz;
+        const float det_y = xx * zz - xz * xz;
+        const float det_z = xx * yy - xy * xy;
+        float dx, dy, dz;
+        if (det_x >= det_y && det_x >= det_z) {
+            dx = 1.0f;
+            dy = (xz * yz - xy * zz) / det_x;
+            dz = (xy * yz - xz * yy) / det_x;
+        } else if (det_y >= det_x && det_y >= det_z) {
+            dx = (yz * xz - xy * zz) / det_y;
+            dy = 1.0f;
+            dz = (xy * xz - yz * xx) / det_y;
+        } else {
+            dx = (yz * xy - xz * yy) / det_z;
+            dy = (xz * xy - yz * xx) / det_z;
+            dz = 1.0f;
+        }
+        const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+        n0 = dx * inv;
+        n1 = dy * inv;
+        n2 = dz * inv;
+    } else if (method == 0) {
+        // calcPlane (tcc:467-520): least squares y = c0 + c1 x + c2 z on the k-set, solved in
+        // fp64 from the query-centred moments; the normal is the cross product of two tangents
+        // that start at the query point, 100 units along x and z (tcc:497-504)
+        const double c = (double)kcount;
+        double A[3][3] = {{c, sx, sz}, {sx, sxx, sxz}, {sz, sxz, szz}};
+        double b[3] = {sy, sxy, syz};
+        solve3(A, b);
+        const double c1 = b[1], c2 = b[2];
+        const double c0 = (double)qy + b[0] - c1 * qx - c2 * qz;
+        const float eps = 100.0f;
+        const float C0 = (float)c0, C1 = (float)c1, C2 = (float)c2;
+        const float z1 = C0 + C1 * (qx + eps) + C2 * qz;
+        const float z2 = C0 + C1 * qx + C2 * (qz + eps);
+        const float d1x = (qx + eps) - qx, d1y = z1 - qy, d1z = qz - qz;
+        const float d2x = qx - qx, d2y = z2 - qy, d2z = (qz + eps) - qz;
+        n0 = d1y * d2z - d1z * d2y;
+        n1 = d1z * d2x - d1x * d2z;
+        n2 = d1x * d2y - d1y * d2x;
+        normalize3(n0, n1, n2);
+    } else {
+        // calcPlaneIPCAExact (tcc:609-673): covariance of {q} + k-set, smallest eigenvector
+        const float cntf = (float)(kcount + 1);
+        const float mx = sx / cntf, my = sy / cntf, mz = sz / cntf;
+        float C[6];
+        C[0] = sxx / cntf - mx * mx;
+        C[1] = sxy / cntf - mx * my;
+        C[2] = sxz / cntf - mx * mz;
+        C[3] = syy / cntf - my * my;
+        C[4] = syz / cntf - my * mz;
+        C[5] = szz / cntf - mz * mz;
+        smallest_eigvec(C, n0, n1, n2);
+    }
+}
+
+// Fused into the kNN pass: the k-set's bounding box test (boundingBoxOK,
+// AdaptiveKSearchSurface.tcc:337-350) and its moments about the query point; the plane fit
+// itself runs in plane_fit_kernel.  Also the sorted ki-prefix for the interpolation.
 template <int P>
 struct SinkNormals {
     // parameters
-    float* nx;
-    float* ny;
-    float* nz;
-    uint8_t* kround;       // per sorted position: adaptive round used (1..5)
+    float* mom;            // [qpos * 9] moments of the k-set about q
     uint32_t* retry;       // per query slot: 1 if the bbox test failed (NULL in the last round)
-    uint64_t* ki_keys;     // scratch [qpos * kiw + j]
-    uint32_t* ki_spos;     // scratch
-    uint32_t* ki_list;     // out [qpos * kiw + r], sorted by key
+    uint32_t* ki_list;     // out [qpos * kiw + j]: the ki nearest, unordered
     int kiw;
-    int round;             // 1..5, k = kn * 2^round
-    int kcount;            // |k-set| (= t0)
-    int method;
-    float fx, fy, fz;
     // per-lane state
     uint32_t qpos_, qi_;
     float qx_, qy_, qz_;
@@ -120,14 +180,11 @@ struct SinkNormals {
             mxz = fmaxf(mxz, z);
         } else {
             const uint32_t slot = atomicAdd(&cnt[g], 1u);
-            if (slot < (uint32_t)kiw) {
-                ki_keys[(size_t)qpos_ * kiw + slot] = key;
-                ki_spos[(size_t)qpos_ * kiw + slot] = spos;
-            }
+            if (slot < (uint32_t)kiw) ki_list[(size_t)qpos_ * kiw + slot] = spos;
         }
     }
 
-    __device__ void finish(int g, int s, bool ok, uint32_t qpos, float qx, float qy, float qz, const uint32_t*,
+    __device__ void finish(int g, int s, bool ok, uint32_t qpos, float, float, float, const uint32_t*,
                            const uint32_t* cnt1)
     {
         sx = group_sum<P>(sx);
@@ -146,98 +203,58 @@ struct SinkNormals {
         mxy = group_max<P>(mxy);
         mxz = group_max<P>(mxz);
         if (ok && s == 0) {
-            // boundingBoxOK, AdaptiveKSearchSurface.tcc:337-350
             const float dx = mxx - mnx, dy = mxy - mny, dz = mxz - mnz;
             const float e = 0.05f;
             const bool bb_ok = dx >= e * dy && dx >= e * dz && dy >= e * dx && dy >= e * dz && dz >= e * dx &&
                                dz >= e * dy;
-            if (!bb_ok && retry) {
-                retry[qi_] = 1u;
-            } else {
-                if (retry) retry[qi_] = 0u;
-                float n0, n1, n2;
-                plane_normal(n0, n1, n2);
-                // flip towards the flip point (tcc:268-274); Normal::operator- renormalises
-                const float ddx = fx - qx, ddy = fy - qy, ddz = fz - qz;
-                if (n0 * ddx + n1 * ddy + n2 * ddz < 0.0f) {
-                    n0 = -n0;
-                    n1 = -n1;
-                    n2 = -n2;
-                    normalize3(n0, n1, n2);
-                }
-                nx[qpos] = n0;
-                ny[qpos] = n1;
-                nz[qpos] = n2;
-                if (kround) kround[qpos] = (uint8_t)round;
-            }
-        }
-        if (kiw > 0 && ok) {
-            group_rank_sort<P>(s, min((int)cnt1[g], kiw), ki_keys + (size_t)qpos * kiw,
-                               ki_spos + (size_t)qpos * kiw, ki_list + (size_t)qpos * kiw);
-        }
-    }
-
-    __device__ void plane_normal(float& n0, float& n1, float& n2) const
-    {
-        if (method == 2) {
-            // calcPlaneIterative (tcc:522-607): second moments about the query point
-            const float xx = sxx, xy = sxy, xz = sxz, yy = syy, yz = syz, zz = szz;
-            const float det_x = yy * zz - yz * yz;
-            const float det_y = xx * zz - xz * xz;
-            const float det_z = xx * yy - xy * xy;
-            float dx, dy, dz;
-            if (det_x >= det_y && det_x >= det_z) {
-                dx = 1.0f;
-                dy = (xz * yz - xy * zz) / det_x;
-                dz = (xy * yz - xz * yy) / det_x;
-            } else if (det_y >= det_x && det_y >= det_z) {
-                dx = (yz * xz - xy * zz) / det_y;
-                dy = 1.0f;
-                dz = (xy * xz - yz * xx) / det_y;
-            } else {
-                dx = (yz * xy - xz * yy) / det_z;
-                dy = (xz * xy - yz * xx) / det_z;
-                dz = 1.0f;
-            }
-            const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
-            n0 = dx * inv;
-            n1 = dy * inv;
-            n2 = dz * inv;
-        } else if (method == 0) {
-            // calcPlane (tcc:467-520): least squares y = c0 + c1 x + c2 z on the k-set,
-            // solved in fp64 from the query-centred moments
-            const double c = (double)kcount;
-            double A[3][3] = {{c, sx, sz}, {sx, sxx, sxz}, {sz, sxz, szz}};
-            double b[3] = {sy, sxy, syz};
-            solve3(A, b);
-            const double c1 = b[1], c2 = b[2];
-            const double c0 = (double)qy_ + b[0] - c1 * qx_ - c2 * qz_;
-            const float eps = 100.0f;
-            const float C0 = (float)c0, C1 = (float)c1, C2 = (float)c2;
-            const float z1 = C0 + C1 * (qx_ + eps) + C2 * qz_;
-            const float z2 = C0 + C1 * qx_ + C2 * (qz_ + eps);
-            const float d1x = (qx_ + eps) - qx_, d1y = z1 - qy_, d1z = qz_ - qz_;
-            const float d2x = qx_ - qx_, d2y = z2 - qy_, d2z = (qz_ + eps) - qz_;
-            n0 = d1y * d2z - d1z * d2y;
-            n1 = d1z * d2x - d1x * d2z;
-            n2 = d1x * d2y - d1y * d2x;
-            normalize3(n0, n1, n2);
-        } else {
-            // calcPlaneIPCAExact (tcc:609-673): covariance of {q} + k-set, smallest
-            // eigenvector.  Moments are taken about q, which the k-set contains.
-            const float cntf = (float)(kcount + 1);
-            const float mx = sx / cntf, my = sy / cntf, mz = sz / cntf;
-            float C[6];
-            C[0] = sxx / cntf - mx * mx;
-            C[1] = sxy / cntf - mx * my;
-            C[2] = sxz / cntf - mx * mz;
-            C[3] = syy / cntf - my * my;
-            C[4] = syz / cntf - my * mz;
-            C[5] = szz / cntf - mz * mz;
-            smallest_eigvec(C, n0, n1, n2);
+            if (retry) retry[qi_] = bb_ok ? 0u : 1u;
+            float* m = mom + (size_t)qpos * 9;
+            m[0] = sx;
+            m[1] = sy;
+            m[2] = sz;
+            m[3] = sxx;
+            m[4] = sxy;
+            m[5] = sxz;
+            m[6] = syy;
+            m[7] = syz;
+            m[8] = szz;
         }
     }
 };
+
+// Plane fit + flip (AdaptiveKSearchSurface.tcc:225-274) for the queries of one adaptive round
+// whose neighbourhood passed boundingBoxOK (or all of them in the last round).
+__global__ __launch_bounds__(256) void plane_fit_kernel(const uint32_t* __restrict__ qsel, int64_t nq,
+                                                        const uint32_t* __restrict__ retry,
+                                                        const float* __restrict__ mom, const float* __restrict__ px,
+                                                        const float* __restrict__ py, const float* __restrict__ pz,
+                                                        int method, int kcount, float fx, float fy, float fz,
+                                                        float* __restrict__ nx, float* __restrict__ ny,
+                                                        float* __restrict__ nz, uint8_t* __restrict__ kround, int round)
+{
+    const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    if (retry && retry[qi]) return;
+    const uint32_t qpos = qsel ? qsel[qi] : (uint32_t)qi;
+    const float qx = px[qpos], qy = py[qpos], qz = pz[qpos];
+    float m[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) m[i] = mom[(size_t)qpos * 9 + i];
+    float n0, n1, n2;
+    plane_from_moments(method, kcount, qx, qy, qz, m, n0, n1, n2);
+    // flip towards the flip point (tcc:268-274); Normal::operator- renormalises
+    const float ddx = fx - qx, ddy = fy - qy, ddz = fz - qz;
+    if (n0 * ddx + n1 * ddy + n2 * ddz < 0.0f) {
+        n0 = -n0;
+        n1 = -n1;
+        n2 = -n2;
+        normalize3(n0, n1, n2);
+    }
+    nx[qpos] = n0;
+    ny[qpos] = n1;
+    nz[qpos] = n2;
+    if (kround) kround[qpos] = (uint8_t)round;
+}
 
 template <int P>
 struct SinkSdf {

@@ -139,6 +139,17 @@ class HipSurface:
                                        int(bool(reset))))
         return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(TIMERS)}
 
+    def kernelStats(self, reset: bool = False) -> dict:
+        """Work counters of the kNN tile kernels, per group (collected while profiling)."""
+        from ._lib import STAT_COUNTERS, STAT_GROUPS
+
+        a = np.zeros(64, np.int64)
+        check(lib().lvrhip_get_kernel_stats(self._h, a.ctypes.data_as(C.c_void_p), int(bool(reset))))
+        return {g: dict(zip(STAT_COUNTERS, (int(x) for x in a[16 * i: 16 * i + 16]))) for i, g in enumerate(STAT_GROUPS)}
+
+    def setTuning(self, lanes_per_query: int = 0, tiles_per_wave: int = 0, r2_margin: float = 0.0) -> None:
+        check(lib().lvrhip_set_tuning(self._h, int(lanes_per_query), int(tiles_per_wave), float(r2_margin)))
+
     def normalsStats(self) -> np.ndarray:
         """Points searched in each adaptive round (k = kn*2, kn*4, ..., kn*32)."""
         a = (C.c_int64 * 5)()
