@@ -54,9 +54,51 @@ const char* lvrhip_last_error(void);
  * kn=10, ki=10, kd=5, flip point (1e6,1e6,1e6), method PCA. */
 int lvrhip_create(const float* xyz, size_t n, int device, lvrhip_t* out);
 
-/* Same, for points already resident in device memory (n*3 floats on `device`).
- * The pointer is only read during the call. */
+/* Same, for points already resident in device memory (n*3 floats on `device`).  The build
+ * reads the pointer during the call; lvrhip_rebuild and graphs captured with rebuild != 0
+ * read it again, so it must stay valid for those. */
 int lvrhip_create_device(const float* d_xyz, size_t n, int device, void* stream, lvrhip_t* out);
+
+/* Batched form: many scans in one handle.  Replaces the per-scan loop of
+ * estimateProjectNormals (util/ScanProjectUtils.cpp:361-410), which builds one
+ * AdaptiveKSearchSurface per scan and runs calculateSurfaceNormals on it.  Scan s owns the
+ * input points [scan_offsets[s], scan_offsets[s+1]) (nscans + 1 non-decreasing offsets,
+ * scan_offsets[0] = 0, at most 65535 scans, empty scans allowed).  Neighbourhoods never
+ * cross scans: every result equals that of a single-scan handle on the scan's points.
+ * All scans share one LBVH (each scan one subtree) and each kernel launch covers every
+ * scan.  Every non-empty scan must hold at least 32*kn points (and ki) when normals are
+ * computed, since the reference's kSearch would otherwise return short neighbourhoods;
+ * kSearch and distances are single-scan only.  The host variant keeps a device copy of
+ * the points; the device variant reads d_xyz like lvrhip_create_device. */
+int lvrhip_create_scans(const float* xyz, const int64_t* scan_offsets, int nscans, int device, lvrhip_t* out);
+int lvrhip_create_scans_device(const float* d_xyz, const int64_t* scan_offsets, int nscans, int device, void* stream,
+                               lvrhip_t* out);
+
+/* Number of scans of the handle (1 for lvrhip_create / lvrhip_create_device). */
+int lvrhip_num_scans(lvrhip_t h, int* nscans);
+
+/* Per-scan flip points, nscans*3 floats (host): estimateProjectNormals sets each scan's
+ * flip point to its scanner origin (ScanProjectUtils.cpp:392).  lvrhip_set_flip_point
+ * switches back to one flip point for all scans. */
+int lvrhip_set_scan_flip_points(lvrhip_t h, const float* xyz);
+
+/* Rebuild the LBVH from the point buffer the handle reads (the device copy, or d_xyz of
+ * the device variants), e.g. after the caller moved the points in place.  Invalidates
+ * the normals.  No lvr2 counterpart (lvr2 builds a new surface). */
+int lvrhip_rebuild(lvrhip_t h);
+
+/* hipGraph form of the whole per-cloud pass (no lvr2 counterpart): captures, on the
+ * handle's stream (or a private stream when it is the null stream), [the LBVH build from
+ * the point buffer if rebuild != 0,] the adaptive normals, the interpolation (ki > 0) and
+ * the copy of the normals into d_out_normals (n*3 floats, input order, device memory).
+ * The capture runs the pass once eagerly first, so the graph allocates nothing; the
+ * graph keeps the parameters (k, method, flip points, tuning) of its capture.
+ * lvrhip_graph_launch replays it without synchronising; lvrhip_synchronize waits for it
+ * and reports kNN capacity errors.  Any other computing call on the handle releases the
+ * graph.  Returns LVRHIP_ESTATE while profiling is on. */
+int lvrhip_graph_capture(lvrhip_t h, float* d_out_normals, int rebuild);
+int lvrhip_graph_launch(lvrhip_t h);
+int lvrhip_synchronize(lvrhip_t h);
 
 /* Replaces ~CudaSurface / CudaSurface::freeGPU (CudaSurface.cu:1259-1264). */
 int lvrhip_destroy(lvrhip_t h);
@@ -158,6 +200,15 @@ int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[64], int reset);
 int lvrhip_set_tuning(lvrhip_t h, int lanes_per_query, int tiles_per_wave, float r2_margin);
 /* ms[i], launches[i] for i < LVRHIP_T_COUNT; either may be NULL.  reset != 0 clears them. */
 int lvrhip_get_timings(lvrhip_t h, double* ms, int64_t* launches, int reset);
+
+/* Autotuner (no lvr2 counterpart): times the adaptive normals pass on a sample of the
+ * cloud (every 16th run of 4096 points in Morton order) for lanes per query in {4, 8},
+ * tiles per wavefront in {4, 8, 16} and radius margins in {1.15, 1.3, 1.6}, keeps the
+ * fastest launch shape and returns it (any output may be NULL).  Invalidates the normals. */
+int lvrhip_autotune(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float* r2_margin);
+
+/* The current launch shape (any output may be NULL). */
+int lvrhip_get_tuning(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float* r2_margin);
 
 #ifdef __cplusplus
 }

@@ -36,6 +36,17 @@ _SIGS = {
     "lvrhip_get_timings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "lvrhip_get_kernel_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "lvrhip_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float]),
+    "lvrhip_create_scans": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "lvrhip_create_scans_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                             C.POINTER(C.c_void_p)]),
+    "lvrhip_num_scans": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "lvrhip_set_scan_flip_points": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lvrhip_rebuild": (C.c_int, [C.c_void_p]),
+    "lvrhip_graph_capture": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "lvrhip_graph_launch": (C.c_int, [C.c_void_p]),
+    "lvrhip_synchronize": (C.c_int, [C.c_void_p]),
+    "lvrhip_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "lvrhip_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
 }
 
 STAT_GROUPS = ("normals_first", "normals_retry", "sdf", "other")

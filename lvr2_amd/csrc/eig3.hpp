@@ -3,7 +3,9 @@
 // smallest_eigvec: eigenvector of the smallest eigenvalue of a symmetric 3x3 matrix, the
 // quantity calcPlaneIPCAExact takes from Eigen::EigenSolver<Matrix3f>
 // (AdaptiveKSearchSurface.tcc:642-659).  Cyclic Jacobi rotations in fp32; when the two
-// smallest eigenvalues are within 1e-3 of the spread the decomposition is redone in fp64.
+// smallest eigenvalues are within EIG_FP64_GAP of the spread the decomposition is redone in
+// fp64.  The fp32 pass is backward stable: its eigenvector error is about
+// eps32 / ((l_mid - l_min) / l_max), 2e-6 rad at the threshold.
 #pragma once
 
 #include "common.hpp"
@@ -68,17 +70,21 @@ __device__ __forceinline__ int smallest_col_dev(T w0, T w1, T w2)
     return 2;
 }
 
-// C: xx xy xz yy yz zz
-__device__ inline void smallest_eigvec(const float* C, float& n0, float& n1, float& n2)
+constexpr float EIG_FP64_GAP = 3e-2f;
+
+// C: xx xy xz yy yz zz (formed in fp64; the fp32 pass sees it rounded)
+__device__ inline void smallest_eigvec(const double* C, float& n0, float& n1, float& n2)
 {
-    float a[3][3] = {{C[0], C[1], C[2]}, {C[1], C[3], C[4]}, {C[2], C[4], C[5]}};
+    float a[3][3] = {{(float)C[0], (float)C[1], (float)C[2]},
+                     {(float)C[1], (float)C[3], (float)C[4]},
+                     {(float)C[2], (float)C[4], (float)C[5]}};
     float v[3][3];
     jacobi3_dev<float>(a, v);
     const float w0 = a[0][0], w1 = a[1][1], w2 = a[2][2];
     const float lo = fminf(w0, fminf(w1, w2)), hi = fmaxf(w0, fmaxf(w1, w2));
     const float mid = w0 + w1 + w2 - lo - hi;
-    if (mid - lo <= 1e-3f * (hi - lo)) {
-        // nearly degenerate: fp64 decomposition of the same matrix
+    if (mid - lo <= EIG_FP64_GAP * (hi - lo)) {
+        // nearly degenerate: fp64 decomposition of the fp64 matrix
         double ad[3][3] = {{C[0], C[1], C[2]}, {C[1], C[3], C[4]}, {C[2], C[4], C[5]}};
         double vd[3][3];
         jacobi3_dev<double>(ad, vd);

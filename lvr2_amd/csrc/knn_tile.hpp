@@ -48,6 +48,8 @@ struct BvhView {
     const BvhNode* nodes;
     int nleaf;
     uint32_t n;
+    const int* roots;         // per scan: root node of its subtree (NULL: one scan, root 0)
+    const uint16_t* scan_of;  // per query position: its scan (NULL: one scan)
 };
 
 struct QueryView {
@@ -56,12 +58,15 @@ struct QueryView {
     const float* qz;
     const uint32_t* qsel;  // optional: tile slot -> query position
     uint32_t nq;           // number of tile slots
+    const uint32_t* nq_dev;  // optional: the number of tile slots, read on the device
 };
 
 struct TileParams {
     int t0, t1;  // targets (t1 <= t0; t1 = 0 for none), already clamped to n
     int tiles_per_wave;
     float r2_init;
+    const float* area_pp;       // optional device value, area per point: first radius^2 =
+                                // area_pp * t0 / pi * r2_margin (else r2_init)
     float r2_margin;            // next tile's radius^2 = margin * largest k-th d^2 of this tile
     int* err;                   // set to a nonzero code on capacity overflow
     unsigned long long* stats;  // optional counters, see KS_* below
@@ -429,23 +434,29 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
     __shared__ TileSmem<Q, BUF> sm;
     const int lane = lane_id();
     const int g = lane / P, s = lane % P;
-    const int root = bv.nleaf > 1 ? 0 : -1;  // internal node 0, or the single leaf (ref -1)
     float r2 = fmaxf(prm.r2_init, 1e-30f);
     WaveStats ws;
     ws.clear();
     ws.timing = prm.stats != nullptr;
     const auto t_wave = ws.tick();
+    const uint32_t nq = qv.nq_dev ? uniform(*qv.nq_dev) : qv.nq;
+    if (prm.area_pp) r2 = fmaxf(uniformf(*prm.area_pp) * (float)prm.t0 / 3.14159265f * prm.r2_margin, 1e-30f);
+    const int64_t ntiles = ((int64_t)nq + Q - 1) / Q;
 
+    // runs of tiles_per_wave consecutive tiles, strided over the grid (a grid sized for an
+    // upper bound of the device-side count leaves the surplus waves idle)
+    for (int64_t run = blockIdx.x; run * prm.tiles_per_wave < ntiles; run += gridDim.x)
     for (int tt = 0; tt < prm.tiles_per_wave; tt++) {
-        const int64_t tile = (int64_t)blockIdx.x * prm.tiles_per_wave + tt;
+        const int64_t tile = run * prm.tiles_per_wave + tt;
         const int64_t qbase = tile * Q;
-        if (qbase >= qv.nq) break;
+        if (tile >= ntiles) break;
         const int64_t qi = qbase + g;
-        const bool valid = qi < qv.nq;
+        const bool valid = qi < nq;
         const uint32_t qpos = valid ? (qv.qsel ? qv.qsel[qi] : (uint32_t)qi) : 0u;
         const float qx = valid ? qv.qx[qpos] : 0.0f;
         const float qy = valid ? qv.qy[qpos] : 0.0f;
         const float qz = valid ? qv.qz[qpos] : 0.0f;
+        const int scan = (valid && bv.scan_of) ? (int)bv.scan_of[qpos] : 0;
         bool active = valid && prm.t0 > 0;
         ws.c[KS_TILES] += 1;
         sink.begin(g, s, qi, qpos, qx, qy, qz);
@@ -456,15 +467,21 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
         float kth_max = 0.0f;
 
         for (int attempt = 0; attempt < MAX_ATTEMPTS; attempt++) {
-            if (ballot(active) == 0) break;
-            // tile box of the pending queries
+            const uint64_t pending = ballot(active);
+            if (pending == 0) break;
+            // One scan per attempt (the scan of the lowest pending lane): neighbourhoods never
+            // cross scans, each scan's points form one subtree of the hierarchy.
+            const int cur = bv.scan_of ? uniform(shfl(scan, __ffsll((long long)pending) - 1)) : 0;
+            const bool act = active && scan == cur;
+            const int root = bv.roots ? uniform(bv.roots[cur]) : (bv.nleaf > 1 ? 0 : -1);
+            // tile box of the pending queries of that scan
             TileBox tb;
-            tb.lx = uniformf(wave_min(active ? qx : INFINITY));
-            tb.ly = uniformf(wave_min(active ? qy : INFINITY));
-            tb.lz = uniformf(wave_min(active ? qz : INFINITY));
-            tb.hx = uniformf(wave_max(active ? qx : -INFINITY));
-            tb.hy = uniformf(wave_max(active ? qy : -INFINITY));
-            tb.hz = uniformf(wave_max(active ? qz : -INFINITY));
+            tb.lx = uniformf(wave_min(act ? qx : INFINITY));
+            tb.ly = uniformf(wave_min(act ? qy : INFINITY));
+            tb.lz = uniformf(wave_min(act ? qz : INFINITY));
+            tb.hx = uniformf(wave_max(act ? qx : -INFINITY));
+            tb.hy = uniformf(wave_max(act ? qy : -INFINITY));
+            tb.hz = uniformf(wave_max(act ? qz : -INFINITY));
             const float R2 = r2;
             const float inv_w = (float)NB / R2;
             ws.c[KS_ATTEMPTS] += 1;
@@ -491,7 +508,7 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
             __syncthreads();
             bool fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
                 auto t0 = ws.tick();
-                const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, active ? R2 : -1.0f);
+                const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, act ? R2 : -1.0f);
                 ws.tock(KS_CY_GLIST, t0);
                 t0 = ws.tick();
                 scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
@@ -514,7 +531,7 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
             {
                 int b, below, cnt;
                 const int total = find_bin<P, Q>(sm.hist, g, s, prm.t0, b, below, cnt);
-                ok = active && total >= prm.t0;
+                ok = act && total >= prm.t0;
 #pragma unroll
                 for (int tau = 0; tau < 2; tau++) {
                     const int t = tau == 0 ? prm.t0 : prm.t1;
@@ -662,7 +679,7 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
             kth_max = fmaxf(kth_max, uniformf(wave_max(ok ? kth : 0.0f)));
             ws.c[KS_RESOLVED] += (unsigned)__popcll(ballot(ok && s == 0));
             active = active && !ok;
-            if (ballot(active) != 0) {
+            if (ballot(act && !ok) != 0) {  // queries of this scan still short of t: grow R
                 float nr2 = R2 * 4.0f;
                 if (!(nr2 > R2) || nr2 < 1e-30f) nr2 = fmaxf(R2 * 4.0f, 1e-30f);
                 if (nr2 > 1e37f) nr2 = INFINITY;

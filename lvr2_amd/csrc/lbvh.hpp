@@ -20,56 +20,10 @@ struct alignas(16) BvhNode {
     float4 a;  // lo0.x lo0.y lo0.z hi0.x
     float4 b;  // hi0.y hi0.z lo1.x lo1.y
     float4 c;  // lo1.z hi1.x hi1.y hi1.z
-    int4 r;    // child0, child1 (>= 0 internal node, < 0 leaf = -1 - id), parent, unused
+    int4 r;    // child0, child1 (>= 0 internal node, < 0 leaf = -1 - id), parent, far end of the leaf range
 };
 
 __device__ __forceinline__ int leaf_ref(int leaf) { return -1 - leaf; }
-
-// ---- bounding box ----------------------------------------------------------------------
-
-__global__ __launch_bounds__(256) void bbox_partial_kernel(const float* __restrict__ xyz, int64_t n,
-                                                           float* __restrict__ part)
-{
-    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            float v = xyz[3 * i + j];
-            mn[j] = fminf(mn[j], v);
-            mx[j] = fmaxf(mx[j], v);
-        }
-    }
-    __shared__ float s[4][6];
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-        mn[j] = wave_min(mn[j]);
-        mx[j] = wave_max(mx[j]);
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int j = 0; j < 3; j++) {
-            s[wave][j] = mn[j];
-            s[wave][3 + j] = mx[j];
-        }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        float v = s[0][threadIdx.x];
-        for (int w = 1; w < 4; w++) v = threadIdx.x < 3 ? fminf(v, s[w][threadIdx.x]) : fmaxf(v, s[w][threadIdx.x]);
-        part[6 * blockIdx.x + threadIdx.x] = v;
-    }
-}
-
-__global__ void bbox_final_kernel(const float* __restrict__ part, int nparts, float* __restrict__ out)
-{
-    if (threadIdx.x < 6) {
-        float v = threadIdx.x < 3 ? INFINITY : -INFINITY;
-        for (int i = 0; i < nparts; i++) {
-            float x = part[6 * i + threadIdx.x];
-            v = threadIdx.x < 3 ? fminf(v, x) : fmaxf(v, x);
-        }
-        out[threadIdx.x] = v;
-    }
-}
 
 // ---- Morton codes ---------------------------------------------------------------------
 
@@ -98,29 +52,158 @@ __device__ __forceinline__ uint64_t morton63(const MortonFrame& f, float x, floa
     return (expand21((uint32_t)u) << 2) | (expand21((uint32_t)v) << 1) | expand21((uint32_t)w);
 }
 
-// Codes of AoS points, values = their index.
-__global__ __launch_bounds__(256) void morton_aos_kernel(const float* __restrict__ xyz, int64_t n, MortonFrame f,
-                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+// ---- scans --------------------------------------------------------------------------------
+// A cloud is one or more scans: input points [off[s], off[s+1]) belong to scan s.  In the
+// sorted arrays each scan starts on a leaf boundary (pstart[s], a multiple of LEAF) and its
+// last leaf is padded with NaN points, so no leaf holds points of two scans.
+
+// largest s in [0, ns) with start[s] <= i (start ascending, empty scans repeat a value)
+__device__ __forceinline__ int scan_of_pos(const int64_t* __restrict__ start, int ns, int64_t i)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// float <-> order-preserving u32 (for atomicMin / atomicMax on boxes)
+__device__ __forceinline__ uint32_t ord_enc(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord_dec(uint32_t e)
+{
+    return __uint_as_float((e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e);
+}
+
+// per scan box (encoded): box[6 s + 0..2] = min, [6 s + 3..5] = max; cleared by box_init_kernel
+__global__ void box_init_kernel(uint32_t* __restrict__ box, int ns)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 6 * ns) box[i] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
+}
+
+constexpr int SB_ITEMS = 16;  // points per lane
+
+// One wave per 64 * SB_ITEMS consecutive input points: reduced in registers and merged with one
+// atomic per coordinate when they belong to one scan, per point otherwise.
+__global__ __launch_bounds__(256) void scan_bbox_kernel(const float* __restrict__ xyz, int64_t n,
+                                                        const int64_t* __restrict__ off, int ns,
+                                                        uint32_t* __restrict__ box)
+{
+    const int lane = lane_id();
+    const int64_t w0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave) * (kWave * SB_ITEMS);
+    if (w0 >= n) return;
+    const int64_t w1 = w0 + kWave * SB_ITEMS < n ? w0 + kWave * SB_ITEMS : n;
+    const int s0 = scan_of_pos(off, ns, w0), s1 = scan_of_pos(off, ns, w1 - 1);
+    if (s0 == s1) {
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int it = 0; it < SB_ITEMS; it++) {
+            const int64_t i = w0 + (int64_t)it * kWave + lane;
+            if (i < w1) {
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const float v = xyz[3 * i + j];
+                    mn[j] = fminf(mn[j], v);
+                    mx[j] = fmaxf(mx[j], v);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            mn[j] = wave_min(mn[j]);
+            mx[j] = wave_max(mx[j]);
+        }
+        if (lane < 3) {
+            atomicMin(&box[6 * s0 + lane], ord_enc(lane == 0 ? mn[0] : lane == 1 ? mn[1] : mn[2]));
+            atomicMax(&box[6 * s0 + 3 + lane], ord_enc(lane == 0 ? mx[0] : lane == 1 ? mx[1] : mx[2]));
+        }
+    } else {
+        for (int it = 0; it < SB_ITEMS; it++) {
+            const int64_t i = w0 + (int64_t)it * kWave + lane;
+            if (i < w1) {
+                const int sc = scan_of_pos(off, ns, i);
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const float v = xyz[3 * i + j];
+                    if (v == v) {
+                        atomicMin(&box[6 * sc + j], ord_enc(v));
+                        atomicMax(&box[6 * sc + 3 + j], ord_enc(v));
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Morton frame of each scan: its box corner, a cubic cell size over the largest extent.
+__global__ void scan_frames_kernel(const uint32_t* __restrict__ box, int ns, MortonFrame* __restrict__ frames)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    float lo[3], ext = 0.0f;
+    for (int j = 0; j < 3; j++) {
+        lo[j] = ord_dec(box[6 * s + j]);
+        const float e = ord_dec(box[6 * s + 3 + j]) - lo[j];
+        ext = e > ext ? e : ext;  // empty scans: NaN / -inf extents leave ext = 0
+    }
+    MortonFrame f;
+    for (int j = 0; j < 3; j++) f.lo[j] = lo[j] == lo[j] && fabsf(lo[j]) < INFINITY ? lo[j] : 0.0f;
+    f.scale = ext > 0.0f ? 2097151.0f / ext : 1.0f;
+    frames[s] = f;
+}
+
+// Sort keys of the input points: the 63-bit Morton code in the scan's own frame; with more
+// than one scan, the scan index in the top 16 bits above the code's leading 48 bits.
+__global__ __launch_bounds__(256) void morton_scan_kernel(const float* __restrict__ xyz, int64_t n,
+                                                          const int64_t* __restrict__ off, int ns,
+                                                          const MortonFrame* __restrict__ frames,
+                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = morton63(f, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    const int sc = ns > 1 ? scan_of_pos(off, ns, i) : 0;
+    const uint64_t m = morton63(frames[sc], xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    keys[i] = ns > 1 ? (((uint64_t)sc << 48) | (m >> 15)) : m;
     vals[i] = (uint32_t)i;
 }
 
-// Sorted SoA copy plus the inverse permutation.
-__global__ __launch_bounds__(256) void gather_sorted_kernel(const float* __restrict__ xyz, int64_t n,
-                                                            const uint32_t* __restrict__ order,
-                                                            float* __restrict__ px, float* __restrict__ py,
-                                                            float* __restrict__ pz, uint32_t* __restrict__ rank)
+// Padded sorted SoA layout, one thread per padded position p: the point (NaN for padding),
+// its input index (~0 for padding), the inverse permutation, the sort key, the list of real
+// positions in sort order, and the scan of each position.
+__global__ __launch_bounds__(256) void gather_padded_kernel(
+    const float* __restrict__ xyz, const int64_t* __restrict__ off, const int64_t* __restrict__ pstart, int ns,
+    int64_t npad, const uint32_t* __restrict__ order, const uint64_t* __restrict__ skeys, float* __restrict__ px,
+    float* __restrict__ py, float* __restrict__ pz, uint32_t* __restrict__ pidx, uint32_t* __restrict__ rank,
+    uint64_t* __restrict__ pcode, uint32_t* __restrict__ qreal, uint16_t* __restrict__ scan_of)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t j = order[i];
-    px[i] = xyz[3 * (int64_t)j];
-    py[i] = xyz[3 * (int64_t)j + 1];
-    pz[i] = xyz[3 * (int64_t)j + 2];
-    if (rank) rank[j] = (uint32_t)i;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npad) return;
+    const int sc = ns > 1 ? scan_of_pos(pstart, ns, p) : 0;
+    const int64_t local = p - pstart[sc];
+    const int64_t cnt = off[sc + 1] - off[sc];
+    if (local < cnt) {
+        const int64_t r = off[sc] + local;
+        const uint32_t j = order[r];
+        px[p] = xyz[3 * (int64_t)j];
+        py[p] = xyz[3 * (int64_t)j + 1];
+        pz[p] = xyz[3 * (int64_t)j + 2];
+        pidx[p] = j;
+        rank[j] = (uint32_t)p;
+        pcode[p] = skeys[r];
+        qreal[r] = (uint32_t)p;
+    } else {
+        px[p] = py[p] = pz[p] = __builtin_nanf("");
+        pidx[p] = ~0u;
+        pcode[p] = cnt > 0 ? skeys[off[sc] + cnt - 1] : 0ull;
+    }
+    if (scan_of) scan_of[p] = (uint16_t)sc;
 }
 
 // Leaf boxes: one lane per point, LEAF lanes per leaf; also the leaf codes.
@@ -132,9 +215,12 @@ __global__ __launch_bounds__(256) void leaf_boxes_kernel(const float* __restrict
 {
     static_assert(LEAF >= 2 && LEAF <= kWave && (LEAF & (LEAF - 1)) == 0, "leaf = power of two lanes");
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = i < n;
+    // padding slots (NaN) are neither in the box nor in the count
+    bool valid = i < n && px[i] == px[i];
     float x = valid ? px[i] : INFINITY, y = valid ? py[i] : INFINITY, z = valid ? pz[i] : INFINITY;
     float X = valid ? px[i] : -INFINITY, Y = valid ? py[i] : -INFINITY, Z = valid ? pz[i] : -INFINITY;
+    const uint64_t vb = ballot(valid);
+    const int cnt = __popcll((vb >> (lane_id() & ~(LEAF - 1))) & ((1ull << LEAF) - 1ull));
 #pragma unroll
     for (int m = LEAF / 2; m >= 1; m >>= 1) {
         x = fminf(x, shfl_xor(x, m));
@@ -146,9 +232,7 @@ __global__ __launch_bounds__(256) void leaf_boxes_kernel(const float* __restrict
     }
     int64_t leaf = i / LEAF;
     if ((i % LEAF) == 0 && leaf < nleaf) {
-        int64_t cnt = n - leaf * LEAF;
-        if (cnt > LEAF) cnt = LEAF;
-        leaf_lo[leaf] = make_float4(x, y, z, __int_as_float((int)cnt));
+        leaf_lo[leaf] = make_float4(x, y, z, __int_as_float(cnt));
         leaf_hi[leaf] = make_float4(X, Y, Z, 0.0f);
         leaf_code[leaf] = codes[i];
     }
@@ -203,6 +287,7 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
     }
     nodes[i].r.x = c0;
     nodes[i].r.y = c1;
+    nodes[i].r.w = j;  // the node covers leaves [min(i, j), max(i, j)]
     if (i == 0) nodes[0].r.z = -1;
 }
 
@@ -248,6 +333,56 @@ __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Root of each scan's subtree: scan s covers leaves [pstart[s] / LEAF, pstart[s+1] / LEAF), a
+// contiguous run sharing the scan's key prefix, which in a radix tree is exactly the leaf
+// range of one node (the lowest ancestor of its first leaf that reaches its last leaf).
+__global__ void scan_roots_kernel(const int64_t* __restrict__ pstart, int ns, const int* __restrict__ leaf_parent,
+                                  const BvhNode* __restrict__ nodes, int* __restrict__ roots)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int ls = (int)(pstart[s] / LEAF), le = (int)(pstart[s + 1] / LEAF) - 1;
+    if (le <= ls) {
+        roots[s] = leaf_ref(ls);  // one leaf (or an empty scan, never queried)
+        return;
+    }
+    int p = leaf_parent[ls];
+    while (p >= 0) {
+        const int4 r = nodes[p].r;
+        const int lo = p < r.w ? p : r.w, hi = p < r.w ? r.w : p;
+        if (lo <= ls && hi >= le) break;
+        p = r.z;
+    }
+    roots[s] = p >= 0 ? p : 0;
+}
+
+// Median of n <= 8192 samples (bitonic sort in LDS, one block of 1024 threads): out = the
+// element of rank n / 2.
+constexpr int MEDIAN_MAX = 8192;
+__global__ __launch_bounds__(1024) void median_kernel(const float* __restrict__ v, int n, float* __restrict__ out)
+{
+    __shared__ float a[MEDIAN_MAX];
+    for (int i = threadIdx.x; i < MEDIAN_MAX; i += blockDim.x) a[i] = i < n ? v[i] : INFINITY;
+    __syncthreads();
+    for (int k = 2; k <= MEDIAN_MAX; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < MEDIAN_MAX; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & k) == 0;
+                    const float x = a[i], y = a[ixj];
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) out[0] = a[n / 2];
 }
 
 }  // namespace lvrhip

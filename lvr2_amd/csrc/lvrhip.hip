@@ -77,12 +77,12 @@ __global__ void leaf_area_kernel(const float4* __restrict__ lo, const float4* __
     out[i] = cnt > 0 ? (m0 * m1) / (float)cnt : 0.0f;
 }
 
-__global__ void morton_queries_kernel(const float* __restrict__ q, int64_t m, MortonFrame f,
+__global__ void morton_queries_kernel(const float* __restrict__ q, int64_t m, const MortonFrame* __restrict__ f,
                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    keys[i] = morton63(f, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+    keys[i] = morton63(*f, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
     vals[i] = (uint32_t)i;
 }
 
@@ -104,9 +104,11 @@ constexpr int CF_ITEMS = 16;
 constexpr int CF_TILE = CF_THREADS * CF_ITEMS;
 
 __global__ __launch_bounds__(CF_THREADS) void flag_block_sums_kernel(const uint32_t* __restrict__ flags, int64_t n,
+                                                                     const uint32_t* __restrict__ n_dev,
                                                                      uint32_t* __restrict__ bsum)
 {
     __shared__ uint32_t ws[CF_THREADS / kWave];
+    if (n_dev) n = (int64_t)*n_dev < n ? (int64_t)*n_dev : n;
     const int64_t base = (int64_t)blockIdx.x * CF_TILE;
     uint32_t c = 0;
     for (int r = 0; r < CF_ITEMS; r++) {
@@ -125,11 +127,13 @@ __global__ __launch_bounds__(CF_THREADS) void flag_block_sums_kernel(const uint3
 }
 
 __global__ __launch_bounds__(CF_THREADS) void compact_scatter_kernel(const uint32_t* __restrict__ flags, int64_t n,
+                                                                     const uint32_t* __restrict__ n_dev,
                                                                      const uint32_t* __restrict__ boff,
                                                                      const uint32_t* __restrict__ qsel,
                                                                      uint32_t* __restrict__ out)
 {
     __shared__ uint32_t wc[CF_THREADS / kWave];
+    if (n_dev) n = (int64_t)*n_dev < n ? (int64_t)*n_dev : n;
     const int64_t base = (int64_t)blockIdx.x * CF_TILE;
     const int wave = threadIdx.x / kWave;
     uint32_t run = boff[blockIdx.x];
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
     __shared__ uint32_t sord[INTERP_THREADS];
     const int grp = threadIdx.x / G, j = threadIdx.x % G, base = grp * G;
     const int64_t i = (int64_t)blockIdx.x * (INTERP_THREADS / G) + grp;
-    const bool valid = i < n;
+    const bool valid = i < n && pidx[i] != ~0u;  // padding slots have no normal
     uint32_t s = 0;
     uint64_t key = ~0ull;
     if (valid && j < kiw) {
@@ -205,10 +209,14 @@ __global__ void gather_normals_kernel(const float* __restrict__ in, const uint32
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t j = pidx[i];
-    nx[i] = in[3 * j];
-    ny[i] = in[3 * j + 1];
-    nz[i] = in[3 * j + 2];
+    const uint32_t j = pidx[i];
+    if (j == ~0u) {
+        nx[i] = ny[i] = nz[i] = 0.0f;
+        return;
+    }
+    nx[i] = in[3 * (int64_t)j];
+    ny[i] = in[3 * (int64_t)j + 1];
+    nz[i] = in[3 * (int64_t)j + 2];
 }
 
 __global__ void scatter_normals_kernel(const float* __restrict__ nx, const float* __restrict__ ny,
@@ -217,10 +225,21 @@ __global__ void scatter_normals_kernel(const float* __restrict__ nx, const float
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t j = pidx[i];
-    out[3 * j] = nx[i];
-    out[3 * j + 1] = ny[i];
-    out[3 * j + 2] = nz[i];
+    const uint32_t j = pidx[i];
+    if (j == ~0u) return;
+    out[3 * (int64_t)j] = nx[i];
+    out[3 * (int64_t)j + 1] = ny[i];
+    out[3 * (int64_t)j + 2] = nz[i];
+}
+
+// Autotuning sample: every `every`-th run of `run` consecutive real positions (sort order).
+__global__ void sample_runs_kernel(const uint32_t* __restrict__ qreal, int64_t m, int run, int every,
+                                   uint32_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int64_t pos = (i / run) * run * every + i % run;
+    out[i] = qreal ? qreal[pos] : (uint32_t)pos;
 }
 
 // Sort each row's t keys (one wave per row) into (index, d^2); pad to k.
@@ -337,21 +356,33 @@ void launch_tile_p(const BvhView& bv, const QueryView& qv, const TileParams& prm
 struct lvrhip_surface {
     int device = 0;
     hipStream_t st = nullptr;
-    int64_t n = 0;
+    int64_t n = 0;     // points
+    int64_t npad = 0;  // sorted positions (each scan padded to a multiple of LEAF)
+    int nscans = 1;
+    std::vector<int64_t> h_off, h_pstart;  // per scan: input offset, sorted start (nscans + 1 entries)
+    int64_t min_scan = 0;                  // smallest non-empty scan
+    const float* src = nullptr;            // device AoS points the build reads
+    DBuf<float> own_src;                   // the copy of host-given points
     int kn = 10, ki = 10, kd = 5, method = LVRHIP_METHOD_PCA;
     float flip[3] = {1e6f, 1e6f, 1e6f};
+    bool scan_flips = false;
     int lanes_per_query = 8;
     int tiles_per_wave = 8;
     float r2_margin = 1.3f;
 
     DBuf<float> px, py, pz;
-    DBuf<uint32_t> pidx, rank;
-    DBuf<float4> leaf_lo, leaf_hi;
+    DBuf<uint32_t> pidx, rank, qreal;
+    DBuf<int64_t> d_off, d_pstart;
+    DBuf<uint16_t> scan_of;
+    DBuf<int> roots;
+    DBuf<float> d_flips;
+    DBuf<uint32_t> sbox;
+    DBuf<MortonFrame> frames;
+    DBuf<float> area;  // median leaf area per point: the first radius guess
+    DBuf<float4> leaf_lo, leaf_hi, nlo, nhi;
     DBuf<BvhNode> nodes;
+    DBuf<int> leaf_parent;
     int nleaf = 0;
-    float bbox[6] = {0, 0, 0, 0, 0, 0};
-    MortonFrame frame{};
-    float area_pp = 0.0f;
 
     DBuf<float> nx, ny, nz, tx, ty, tz;
     bool has_normals = false;
@@ -359,12 +390,27 @@ struct lvrhip_surface {
     int kiw = 0;
     bool ki_valid = false;
     DBuf<uint8_t> kround;
-    DBuf<float> mom;  // per point: moments of its k-set about the point (9 floats)
+    DBuf<float> mom;  // per position: moments of its k-set about the point (9 floats)
     DBuf<uint32_t> s_bsum;
+    DBuf<uint32_t> d_cnt;  // [r] = queries of adaptive round r (r = 2..5), counted on the device
+    int64_t nq_first = 0;  // queries of the first round of the last normals pass
     int64_t n_round[5] = {0, 0, 0, 0, 0};
     DBuf<int> err;
     Profiler prof;
     DBuf<unsigned long long> kstats;  // 4 groups x KS_COUNT
+
+    hipStream_t gst = nullptr;  // private stream for graphs when the handle's stream is the null stream
+    bool own_gst = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipStream_t graph_stream = nullptr;
+
+    ~lvrhip_surface()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (own_gst && gst) (void)hipStreamDestroy(gst);
+    }
 
     unsigned long long* kstats_ptr(int group)
     {
@@ -378,7 +424,7 @@ struct lvrhip_surface {
 
     // scratch
     DBuf<uint64_t> s_k0, s_k1;
-    DBuf<uint32_t> s_v0, s_v1, s_hist, s_flags, s_scan, s_list0, s_list1, s_spos, s_ord;
+    DBuf<uint32_t> s_v0, s_v1, s_hist, s_flags, s_list0, s_list1, s_spos, s_ord, s_sample;
     DBuf<float> s_qx, s_qy, s_qz, s_f;
     DBuf<int32_t> s_idx;
 
@@ -393,28 +439,60 @@ struct lvrhip_surface {
         v.leaf_hi = leaf_hi.p;
         v.nodes = nodes.p;
         v.nleaf = nleaf;
-        v.n = (uint32_t)n;
+        v.n = (uint32_t)npad;
+        v.roots = nscans > 1 ? roots.p : nullptr;
+        v.scan_of = nscans > 1 ? scan_of.p : nullptr;
         return v;
     }
 
-    float r2_init(int k) const
+    TileParams params(int t0, int t1, int stats_group) const
     {
-        float r2 = area_pp * (float)k / 3.14159265f * r2_margin;
-        return r2 > 1e-30f ? r2 : 1e-30f;
+        TileParams prm{};
+        prm.t0 = t0;
+        prm.t1 = t1;
+        prm.tiles_per_wave = tiles_per_wave;
+        prm.r2_init = 0.0f;
+        prm.area_pp = area.p;
+        prm.r2_margin = r2_margin;
+        prm.err = err.p;
+        prm.stats = nullptr;
+        (void)stats_group;
+        return prm;
     }
 
-    void check_err()
+    // The round-1 query list: every real sorted position (NULL = identity, one scan).
+    const uint32_t* all_queries() const { return nscans > 1 ? qreal.p : nullptr; }
+
+    // A neighbourhood size as the reference's kSearch returns it: at most the points of the
+    // scan.  Scans of a multi-scan cloud must each hold at least k points.
+    int clamp_k(int64_t k) const
+    {
+        if (nscans == 1) return (int)std::min<int64_t>(k, n);
+        if (k > min_scan)
+            throw Error(LVRHIP_EINVAL, "every scan needs at least " + std::to_string(k) +
+                                           " points for this k (smallest scan: " + std::to_string(min_scan) + ")");
+        return (int)k;
+    }
+
+    void read_err_and_counts()
     {
         int e = 0;
+        uint32_t c[8] = {};
         LVRHIP_CHECK(hipMemcpyAsync(&e, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        if (d_cnt.p) LVRHIP_CHECK(hipMemcpyAsync(c, d_cnt.p, sizeof(c), hipMemcpyDeviceToHost, st));
         LVRHIP_CHECK(hipStreamSynchronize(st));
+        if (d_cnt.p) {
+            n_round[0] = nq_first;
+            for (int r = 2; r <= 5; r++) n_round[r - 1] = c[r];
+        }
         if (e != 0) {
-            int z = 0;
-            LVRHIP_CHECK(hipMemcpyAsync(err.p, &z, sizeof(int), hipMemcpyHostToDevice, st));
+            fill_u32_kernel<<<1, 64, 0, st>>>((uint32_t*)err.p, 1, 0u);
+            LVRHIP_LAUNCH_CHECK();
             LVRHIP_CHECK(hipStreamSynchronize(st));
             throw Error(LVRHIP_ECAPACITY, e == 5 ? "kNN traversal stack overflow" : "kNN radius search did not converge");
         }
     }
+    void check_err() { read_err_and_counts(); }
 
     template <template <int> class SinkT>
     void tile(const QueryView& qv, const TileParams& prm, int P, SinkT<2> s2, SinkT<4> s4, SinkT<8> s8)
@@ -427,133 +505,163 @@ struct lvrhip_surface {
         }
     }
 
-    void build(const float* d_xyz)
+    // Scan table and every buffer the build writes (sized once, so that the build itself
+    // allocates nothing and can be captured in a graph).
+    void init(const float* d_xyz, const int64_t* offsets, int ns)
+    {
+        nscans = ns;
+        h_off.assign(offsets, offsets + ns + 1);
+        h_pstart.assign(ns + 1, 0);
+        min_scan = 0;
+        for (int s = 0; s < ns; s++) {
+            const int64_t c = h_off[s + 1] - h_off[s];
+            h_pstart[s + 1] = h_pstart[s] + ceil_div(c, LEAF) * LEAF;
+            if (c > 0 && (min_scan == 0 || c < min_scan)) min_scan = c;
+        }
+        n = h_off[ns];
+        npad = h_pstart[ns];
+        src = d_xyz;
+        err.reserve(1);
+        fill_u32_kernel<<<1, 64, 0, st>>>((uint32_t*)err.p, 1, 0u);
+        LVRHIP_LAUNCH_CHECK();
+        if (n == 0) return;
+        d_off.reserve(ns + 1);
+        d_pstart.reserve(ns + 1);
+        LVRHIP_CHECK(hipMemcpyAsync(d_off.p, h_off.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        LVRHIP_CHECK(
+            hipMemcpyAsync(d_pstart.p, h_pstart.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        nleaf = (int)(npad / LEAF);
+        const int nint = std::max(nleaf - 1, 1);
+        sbox.reserve(6 * (size_t)ns);
+        frames.reserve(ns);
+        area.reserve(1);
+        s_k0.reserve(std::max(n, (int64_t)nleaf));
+        s_k1.reserve(std::max(n, npad));
+        s_v0.reserve(n);
+        s_v1.reserve(n);
+        s_hist.reserve((size_t)RS_RADIX * ceil_div(n, RS_TILE));
+        px.reserve(npad);
+        py.reserve(npad);
+        pz.reserve(npad);
+        pidx.reserve(npad);
+        rank.reserve(n);
+        qreal.reserve(n);
+        if (ns > 1) {
+            scan_of.reserve(npad);
+            roots.reserve(ns);
+        }
+        leaf_lo.reserve(nleaf);
+        leaf_hi.reserve(nleaf);
+        leaf_parent.reserve(nleaf);
+        nodes.reserve(nint);
+        nlo.reserve(nint);
+        nhi.reserve(nint);
+        s_flags.reserve(std::max<int64_t>(nint, n));
+        s_f.reserve(MEDIAN_MAX);
+    }
+
+    // LBVH build from `src`, entirely on the device (no host synchronisation).
+    void build()
     {
         if (n <= 0) return;
         const int64_t N = n;
-        // bounding box
-        const int nparts = 256;
-        s_f.reserve(6 * nparts + 6);
-        bbox_partial_kernel<<<nparts, 256, 0, st>>>(d_xyz, N, s_f.p);
+        const int ns = nscans;
+        box_init_kernel<<<(unsigned)ceil_div(6 * ns, 256), 256, 0, st>>>(sbox.p, ns);
         LVRHIP_LAUNCH_CHECK();
-        bbox_final_kernel<<<1, 64, 0, st>>>(s_f.p, nparts, s_f.p + 6 * nparts);
+        const int64_t waves = ceil_div(N, (int64_t)kWave * SB_ITEMS);
+        scan_bbox_kernel<<<(unsigned)ceil_div(waves, 4), 256, 0, st>>>(src, N, d_off.p, ns, sbox.p);
         LVRHIP_LAUNCH_CHECK();
-        LVRHIP_CHECK(hipMemcpyAsync(bbox, s_f.p + 6 * nparts, 6 * sizeof(float), hipMemcpyDeviceToHost, st));
-        LVRHIP_CHECK(hipStreamSynchronize(st));
-        float ext = std::max(bbox[3] - bbox[0], std::max(bbox[4] - bbox[1], bbox[5] - bbox[2]));
-        frame.lo[0] = bbox[0];
-        frame.lo[1] = bbox[1];
-        frame.lo[2] = bbox[2];
-        frame.scale = ext > 0.0f ? 2097151.0f / ext : 1.0f;
-        // Morton sort
-        s_k0.reserve(N);
-        s_k1.reserve(N);
-        pidx.reserve(N);
-        s_v1.reserve(N);
-        s_hist.reserve((size_t)RS_RADIX * ceil_div(N, RS_TILE));
+        scan_frames_kernel<<<(unsigned)ceil_div(ns, 256), 256, 0, st>>>(sbox.p, ns, frames.p);
+        LVRHIP_LAUNCH_CHECK();
+        // sort by (scan, Morton code)
         const unsigned gb = (unsigned)ceil_div(N, 256);
-        morton_aos_kernel<<<gb, 256, 0, st>>>(d_xyz, N, frame, s_k0.p, pidx.p);
+        morton_scan_kernel<<<gb, 256, 0, st>>>(src, N, d_off.p, ns, frames.p, s_k0.p, s_v0.p);
         LVRHIP_LAUNCH_CHECK();
-        radix_sort_pairs(s_k0.p, pidx.p, s_k1.p, s_v1.p, N, s_hist.p, st);
-        px.reserve(N);
-        py.reserve(N);
-        pz.reserve(N);
-        rank.reserve(N);
-        gather_sorted_kernel<<<gb, 256, 0, st>>>(d_xyz, N, pidx.p, px.p, py.p, pz.p, rank.p);
+        radix_sort_pairs(s_k0.p, s_v0.p, s_k1.p, s_v1.p, N, s_hist.p, st);
+        const unsigned gp = (unsigned)ceil_div(npad, 256);
+        gather_padded_kernel<<<gp, 256, 0, st>>>(src, d_off.p, d_pstart.p, ns, npad, s_v0.p, s_k0.p, px.p, py.p, pz.p,
+                                                 pidx.p, rank.p, s_k1.p, qreal.p, ns > 1 ? scan_of.p : nullptr);
         LVRHIP_LAUNCH_CHECK();
         // leaves and hierarchy
-        nleaf = (int)ceil_div(N, LEAF);
-        leaf_lo.reserve(nleaf);
-        leaf_hi.reserve(nleaf);
-        s_k1.reserve(nleaf);
-        leaf_boxes_kernel<<<gb, 256, 0, st>>>(px.p, py.p, pz.p, N, s_k0.p, nleaf, leaf_lo.p, leaf_hi.p, s_k1.p);
+        leaf_boxes_kernel<<<gp, 256, 0, st>>>(px.p, py.p, pz.p, npad, s_k1.p, nleaf, leaf_lo.p, leaf_hi.p, s_k0.p);
         LVRHIP_LAUNCH_CHECK();
         const int nint = std::max(nleaf - 1, 1);
-        nodes.reserve(nint);
-        DBuf<int> leaf_parent;
-        leaf_parent.reserve(nleaf);
         fill_u32_kernel<<<(unsigned)ceil_div(nleaf, 256), 256, 0, st>>>((uint32_t*)leaf_parent.p, nleaf, ~0u);
         LVRHIP_LAUNCH_CHECK();
-        DBuf<float4> nlo, nhi;
-        nlo.reserve(nint);
-        nhi.reserve(nint);
-        s_flags.reserve(nint);
         fill_u32_kernel<<<(unsigned)ceil_div(nint, 256), 256, 0, st>>>(s_flags.p, nint, 0u);
         LVRHIP_LAUNCH_CHECK();
         if (nleaf > 1) {
-            karras_kernel<<<(unsigned)ceil_div(nleaf - 1, 256), 256, 0, st>>>(s_k1.p, nleaf, nodes.p, leaf_parent.p);
+            karras_kernel<<<(unsigned)ceil_div(nleaf - 1, 256), 256, 0, st>>>(s_k0.p, nleaf, nodes.p, leaf_parent.p);
             LVRHIP_LAUNCH_CHECK();
             refit_kernel<<<(unsigned)ceil_div(nleaf, 256), 256, 0, st>>>(nleaf, leaf_parent.p, leaf_lo.p, leaf_hi.p,
                                                                         nodes.p, nlo.p, nhi.p, s_flags.p);
             LVRHIP_LAUNCH_CHECK();
         }
+        if (ns > 1) {
+            scan_roots_kernel<<<(unsigned)ceil_div(ns, 256), 256, 0, st>>>(d_pstart.p, ns, leaf_parent.p, nodes.p,
+                                                                          roots.p);
+            LVRHIP_LAUNCH_CHECK();
+        }
         // density scale for the first radius guess: median leaf area per point
-        const int nsample = std::min(nleaf, 8192);
+        const int nsample = std::min(nleaf, MEDIAN_MAX);
         const int stride = std::max(1, nleaf / nsample);
-        s_f.reserve(std::max<size_t>(nsample, 6 * nparts + 6));
         leaf_area_kernel<<<(unsigned)ceil_div(nsample, 256), 256, 0, st>>>(leaf_lo.p, leaf_hi.p, nleaf, stride,
                                                                           nsample, s_f.p);
         LVRHIP_LAUNCH_CHECK();
-        std::vector<float> a(nsample);
-        LVRHIP_CHECK(hipMemcpyAsync(a.data(), s_f.p, nsample * sizeof(float), hipMemcpyDeviceToHost, st));
-        LVRHIP_CHECK(hipStreamSynchronize(st));
-        std::nth_element(a.begin(), a.begin() + nsample / 2, a.end());
-        area_pp = a[nsample / 2];
-        err.reserve(1);
-        int z = 0;
-        LVRHIP_CHECK(hipMemcpyAsync(err.p, &z, sizeof(int), hipMemcpyHostToDevice, st));
-        LVRHIP_CHECK(hipStreamSynchronize(st));
+        median_kernel<<<1, 1024, 0, st>>>(s_f.p, nsample, area.p);
+        LVRHIP_LAUNCH_CHECK();
+        fill_u32_kernel<<<1, 64, 0, st>>>((uint32_t*)err.p, 1, 0u);
+        LVRHIP_LAUNCH_CHECK();
     }
 
-    // out = the entries i < nq with flags[i] != 0 (qsel[i] or i), in order; *count_dev = count
-    void compact_flags(const uint32_t* flags, int64_t nq, const uint32_t* qsel, uint32_t* out, uint32_t* count_dev)
+    // out = the entries i < min(nq, *nq_dev) with flags[i] != 0 (qsel[i] or i), in order;
+    // *count_dev = their count
+    void compact_flags(const uint32_t* flags, int64_t nq, const uint32_t* nq_dev, const uint32_t* qsel, uint32_t* out,
+                       uint32_t* count_dev)
     {
-        const int64_t nb = ceil_div(nq, CF_TILE);
+        const int64_t nb = std::max<int64_t>(ceil_div(nq, CF_TILE), 1);
         s_bsum.reserve(nb + 1);
-        flag_block_sums_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, s_bsum.p);
+        flag_block_sums_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, nq_dev, s_bsum.p);
         LVRHIP_LAUNCH_CHECK();
         scan_exclusive_kernel<<<1, 1024, 0, st>>>(s_bsum.p, nb, count_dev);
         LVRHIP_LAUNCH_CHECK();
-        compact_scatter_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, s_bsum.p, qsel, out);
+        compact_scatter_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, nq_dev, s_bsum.p, qsel, out);
         LVRHIP_LAUNCH_CHECK();
     }
 
-    void calculate_normals()
+    // Buffers of a normals pass (host side only: no GPU work).
+    void prepare_normals()
     {
-        const int64_t N = n;
-        has_normals = false;
-        ki_valid = false;
-        for (auto& c : n_round) c = 0;
-        if (N == 0) {
-            has_normals = true;
-            return;
-        }
-        nx.reserve(N);
-        ny.reserve(N);
-        nz.reserve(N);
-        kround.reserve(N);
-        mom.reserve((size_t)N * 9);
-        kiw = (ki > 0 && ki <= 2 * kn) ? (int)std::min<int64_t>(ki, N) : 0;
-        if (kiw > 0) ki_list.reserve((size_t)N * kiw);
-        s_flags.reserve(N);
-        s_scan.reserve(N + 1);
-        s_list0.reserve(N);
-        s_list1.reserve(N);
-        const uint32_t* qsel = nullptr;
-        int64_t nq = N;
+        kiw = (ki > 0 && ki <= 2 * kn) ? clamp_k(ki) : 0;
+        for (int r = 1; r <= 5; r++) (void)clamp_k((int64_t)kn << r);
+        nx.reserve(npad);
+        ny.reserve(npad);
+        nz.reserve(npad);
+        kround.reserve(npad);
+        mom.reserve((size_t)npad * 9);
+        if (kiw > 0) ki_list.reserve((size_t)npad * kiw);
+        s_flags.reserve(n);
+        s_list0.reserve(n);
+        s_list1.reserve(n);
+        d_cnt.reserve(8);
+    }
+
+    // calculateSurfaceNormals' adaptive rounds (AdaptiveKSearchSurface.tcc:187-284) for the
+    // queries q0[0, nq0) (NULL: all points): round r searches k = kn * 2^r for the queries the
+    // previous round's bounding-box test rejected; the counts stay on the device, so the
+    // sequence enqueues without host synchronisation.
+    void enqueue_normals(const uint32_t* q0, int64_t nq0)
+    {
         uint32_t* lists[2] = {s_list0.p, s_list1.p};
+        nq_first = nq0;
+        if (nq0 == 0) return;
         for (int r = 1; r <= 5; r++) {
-            const int64_t k = (int64_t)kn << r;
-            const int t0 = (int)std::min<int64_t>(k, N);
-            TileParams prm{};
-            prm.t0 = t0;
-            prm.t1 = (r == 1) ? kiw : 0;
-            prm.tiles_per_wave = tiles_per_wave;
-            prm.r2_init = r2_init(t0);
-            prm.r2_margin = r2_margin;
-            prm.err = err.p;
+            const int t0 = clamp_k((int64_t)kn << r);
+            TileParams prm = params(t0, r == 1 ? kiw : 0, 0);
             prm.stats = kstats_ptr(r == 1 ? 0 : 1);
-            QueryView qv{px.p, py.p, pz.p, qsel, (uint32_t)nq};
+            const uint32_t* qsel = r == 1 ? q0 : lists[(r - 1) & 1];
+            const uint32_t* nq_dev = r == 1 ? nullptr : d_cnt.p + r;
+            QueryView qv{px.p, py.p, pz.p, qsel, (uint32_t)nq0, nq_dev};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
                 SinkNormals<P> s{};
@@ -566,48 +674,62 @@ struct lvrhip_surface {
             const int tok = prof.start(r == 1 ? LVRHIP_T_NORMALS_FIRST : LVRHIP_T_NORMALS_RETRY, st);
             tile<SinkNormals>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
                               mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
-            plane_fit_kernel<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(
-                qsel, nq, r < 5 ? s_flags.p : nullptr, mom.p, px.p, py.p, pz.p, method, t0, flip[0], flip[1], flip[2],
-                nx.p, ny.p, nz.p, kround.p, r);
+            PlaneFitParams pf{};
+            pf.qsel = qsel;
+            pf.nq = nq0;
+            pf.nq_dev = nq_dev;
+            pf.retry = r < 5 ? s_flags.p : nullptr;
+            pf.mom = mom.p;
+            pf.px = px.p;
+            pf.py = py.p;
+            pf.pz = pz.p;
+            pf.method = method;
+            pf.kcount = t0;
+            pf.fx = flip[0];
+            pf.fy = flip[1];
+            pf.fz = flip[2];
+            pf.flips = (scan_flips && nscans > 1) ? d_flips.p : nullptr;
+            pf.scan_of = nscans > 1 ? scan_of.p : nullptr;
+            pf.nx = nx.p;
+            pf.ny = ny.p;
+            pf.nz = nz.p;
+            pf.kround = kround.p;
+            pf.round = r;
+            plane_fit_kernel<<<(unsigned)ceil_div(nq0, 256), 256, 0, st>>>(pf);
             LVRHIP_LAUNCH_CHECK();
-            n_round[r - 1] = nq;
-            if (r == 5) {
-                prof.stop(tok, st);
-                break;
-            }
-            // failing queries -> next list
-            uint32_t* out = lists[r & 1];
-            compact_flags(s_flags.p, nq, qsel, out, s_scan.p + N);
+            // rejected queries -> the next round's list and count
+            if (r < 5) compact_flags(s_flags.p, nq0, nq_dev, qsel, lists[r & 1], d_cnt.p + r + 1);
             prof.stop(tok, st);
-            uint32_t cnt = 0;
-            LVRHIP_CHECK(hipMemcpyAsync(&cnt, s_scan.p + N, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-            LVRHIP_CHECK(hipStreamSynchronize(st));
-            if (cnt == 0) break;
-            qsel = out;
-            nq = cnt;
         }
-        check_err();
+    }
+
+    void calculate_normals()
+    {
+        drop_graph();
+        has_normals = false;
+        ki_valid = false;
+        for (auto& c : n_round) c = 0;
+        if (n == 0) {
+            has_normals = true;
+            return;
+        }
+        prepare_normals();
+        enqueue_normals(all_queries(), n);
+        read_err_and_counts();
         prof.collect();
         has_normals = true;
         ki_valid = kiw > 0;
     }
 
-    void ensure_ki_list()
+    // the ki list when the first normals round could not supply it (ki > 2 kn)
+    void enqueue_ki_list()
     {
-        const int64_t N = n;
-        const int kw = (int)std::min<int64_t>(ki, N);
-        if (ki_valid && kiw == kw) return;
+        const int kw = clamp_k(ki);
         kiw = kw;
-        ki_list.reserve((size_t)N * kw);
-        TileParams prm{};
-        prm.t0 = kw;
-        prm.t1 = 0;
-        prm.tiles_per_wave = tiles_per_wave;
-        prm.r2_init = r2_init(kw);
-        prm.r2_margin = r2_margin;
-        prm.err = err.p;
+        ki_list.reserve((size_t)npad * kw);
+        TileParams prm = params(kw, 0, 3);
         prm.stats = kstats_ptr(3);
-        QueryView qv{px.p, py.p, pz.p, nullptr, (uint32_t)N};
+        QueryView qv{px.p, py.p, pz.p, all_queries(), (uint32_t)n, nullptr};
         auto mk = [&](auto tag) {
             constexpr int P = decltype(tag)::value;
             SinkKiOnly<P> s{};
@@ -617,33 +739,48 @@ struct lvrhip_surface {
         };
         tile<SinkKiOnly>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
                          mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
-        check_err();
-        ki_valid = true;
+    }
+
+    void check_interpolate() const
+    {
+        if (std::min<int64_t>(ki, nscans > 1 ? min_scan : n) > 64)
+            throw Error(LVRHIP_EINVAL, "ki above 64 is not supported");
+    }
+
+    // interpolateSurfaceNormals (tcc:293-335) from nx/ny/nz into ox/oy/oz
+    void enqueue_interpolate(float* ox, float* oy, float* oz)
+    {
+        const int64_t R = npad;
+        const int tok = prof.start(LVRHIP_T_INTERPOLATE, st);
+        if (kiw <= 16) {
+            interpolate_kernel<16><<<(unsigned)ceil_div(R, INTERP_THREADS / 16), INTERP_THREADS, 0, st>>>(
+                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, R, ox, oy, oz);
+        } else if (kiw <= 32) {
+            interpolate_kernel<32><<<(unsigned)ceil_div(R, INTERP_THREADS / 32), INTERP_THREADS, 0, st>>>(
+                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, R, ox, oy, oz);
+        } else {
+            interpolate_kernel<64><<<(unsigned)ceil_div(R, INTERP_THREADS / 64), INTERP_THREADS, 0, st>>>(
+                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, R, ox, oy, oz);
+        }
+        LVRHIP_LAUNCH_CHECK();
+        prof.stop(tok, st);
     }
 
     void interpolate_normals()
     {
         if (!has_normals) throw Error(LVRHIP_ESTATE, "interpolate_normals called before calculate_normals");
+        drop_graph();
         if (ki <= 0 || n == 0) return;
-        if (std::min<int64_t>(ki, n) > 64) throw Error(LVRHIP_EINVAL, "ki above 64 is not supported");
-        ensure_ki_list();
-        const int64_t N = n;
-        tx.reserve(N);
-        ty.reserve(N);
-        tz.reserve(N);
-        const int tok = prof.start(LVRHIP_T_INTERPOLATE, st);
-        if (kiw <= 16) {
-            interpolate_kernel<16><<<(unsigned)ceil_div(N, INTERP_THREADS / 16), INTERP_THREADS, 0, st>>>(
-                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, N, tx.p, ty.p, tz.p);
-        } else if (kiw <= 32) {
-            interpolate_kernel<32><<<(unsigned)ceil_div(N, INTERP_THREADS / 32), INTERP_THREADS, 0, st>>>(
-                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, N, tx.p, ty.p, tz.p);
-        } else {
-            interpolate_kernel<64><<<(unsigned)ceil_div(N, INTERP_THREADS / 64), INTERP_THREADS, 0, st>>>(
-                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, N, tx.p, ty.p, tz.p);
+        check_interpolate();
+        if (!(ki_valid && kiw == clamp_k(ki))) {
+            enqueue_ki_list();
+            check_err();
+            ki_valid = true;
         }
-        LVRHIP_LAUNCH_CHECK();
-        prof.stop(tok, st);
+        tx.reserve(npad);
+        ty.reserve(npad);
+        tz.reserve(npad);
+        enqueue_interpolate(tx.p, ty.p, tz.p);
         if (prof.on) prof.collect();
         std::swap(nx.p, tx.p);
         std::swap(nx.cap, tx.cap);
@@ -651,6 +788,172 @@ struct lvrhip_surface {
         std::swap(ny.cap, ty.cap);
         std::swap(nz.p, tz.p);
         std::swap(nz.cap, tz.cap);
+    }
+
+    void scatter_normals(const float* ax, const float* ay, const float* az, float* d_out)
+    {
+        scatter_normals_kernel<<<(unsigned)ceil_div(npad, 256), 256, 0, st>>>(ax, ay, az, pidx.p, npad, d_out);
+        LVRHIP_LAUNCH_CHECK();
+    }
+
+    // The whole per-cloud pipeline, enqueue only: [LBVH build,] adaptive normals,
+    // interpolation, normals to d_out in input order.
+    void enqueue_pipeline(float* d_out, bool rebuild)
+    {
+        prepare_normals();
+        if (rebuild) build();
+        enqueue_normals(all_queries(), n);
+        if (ki > 0) {
+            if (kiw != clamp_k(ki)) enqueue_ki_list();
+            tx.reserve(npad);
+            ty.reserve(npad);
+            tz.reserve(npad);
+            enqueue_interpolate(tx.p, ty.p, tz.p);
+            scatter_normals(tx.p, ty.p, tz.p, d_out);
+        } else {
+            scatter_normals(nx.p, ny.p, nz.p, d_out);
+        }
+    }
+
+    hipStream_t graph_st()
+    {
+        if (st) return st;
+        if (!gst) {
+            LVRHIP_CHECK(hipStreamCreateWithFlags(&gst, hipStreamNonBlocking));
+            own_gst = true;
+        }
+        return gst;
+    }
+
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+
+    void graph_capture(float* d_out, bool rebuild)
+    {
+        if (prof.on) throw Error(LVRHIP_ESTATE, "switch profiling off before capturing a graph");
+        if (n == 0) throw Error(LVRHIP_EINVAL, "empty point cloud");
+        if (ki > 0) check_interpolate();
+        prepare_normals();
+        drop_graph();
+        hipStream_t saved = st;
+        st = graph_st();
+        try {
+            // one eager pass sizes every buffer, so the captured pass allocates nothing
+            enqueue_pipeline(d_out, rebuild);
+            read_err_and_counts();
+            LVRHIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_pipeline(d_out, rebuild);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            LVRHIP_CHECK(hipStreamEndCapture(st, &graph));
+            LVRHIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            graph_stream = st;
+        } catch (...) {
+            st = saved;
+            throw;
+        }
+        st = saved;
+        has_normals = false;  // the handle's eager normals are not the graph's output
+        ki_valid = false;
+    }
+
+    void graph_launch()
+    {
+        if (!gexec) throw Error(LVRHIP_ESTATE, "no graph captured");
+        LVRHIP_CHECK(hipGraphLaunch(gexec, graph_stream));
+    }
+
+    void synchronize()
+    {
+        hipStream_t saved = st;
+        if (gexec) st = graph_stream;
+        try {
+            read_err_and_counts();
+        } catch (...) {
+            st = saved;
+            throw;
+        }
+        st = saved;
+        if (!gexec) LVRHIP_CHECK(hipStreamSynchronize(st));
+    }
+
+    // Project autotuner: times the adaptive normals pass on a sample (every 16th run of 4096
+    // consecutive points in sort order) for each (lanes per query, tiles per wave, radius
+    // margin) and keeps the fastest.
+    void autotune(int* outP, int* outT, float* outM)
+    {
+        if (n == 0) return;
+        if (prof.on) throw Error(LVRHIP_ESTATE, "switch profiling off before autotuning");
+        drop_graph();
+        prepare_normals();
+        const int RUN = 4096, EVERY = 16;
+        const int64_t runs = ceil_div(n, (int64_t)RUN);
+        int64_t m = 0;
+        for (int64_t r = 0; r < runs; r += EVERY) m += std::min<int64_t>(RUN, n - r * RUN);
+        s_sample.reserve(m);
+        sample_runs_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(all_queries(), m, RUN, EVERY, s_sample.p);
+        LVRHIP_LAUNCH_CHECK();
+        hipEvent_t a, b;
+        LVRHIP_CHECK(hipEventCreate(&a));
+        LVRHIP_CHECK(hipEventCreate(&b));
+        const int Ps[2] = {4, 8};
+        const int Ts[3] = {4, 8, 16};
+        const float Ms[3] = {1.15f, 1.3f, 1.6f};
+        float best = INFINITY;
+        int bp = lanes_per_query, bt = tiles_per_wave;
+        float bm = r2_margin;
+        try {
+            enqueue_normals(s_sample.p, m);  // warm-up
+            read_err_and_counts();
+            for (int P : Ps)
+                for (int T : Ts)
+                    for (float M : Ms) {
+                        lanes_per_query = P;
+                        tiles_per_wave = T;
+                        r2_margin = M;
+                        float t = INFINITY;
+                        for (int rep = 0; rep < 2; rep++) {
+                            LVRHIP_CHECK(hipEventRecord(a, st));
+                            enqueue_normals(s_sample.p, m);
+                            LVRHIP_CHECK(hipEventRecord(b, st));
+                            read_err_and_counts();
+                            float ms = 0.0f;
+                            LVRHIP_CHECK(hipEventElapsedTime(&ms, a, b));
+                            t = std::min(t, ms);
+                        }
+                        if (t < best) {
+                            best = t;
+                            bp = P;
+                            bt = T;
+                            bm = M;
+                        }
+                    }
+        } catch (...) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            throw;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        lanes_per_query = bp;
+        tiles_per_wave = bt;
+        r2_margin = bm;
+        has_normals = false;
+        ki_valid = false;
+        for (auto& c : n_round) c = 0;
+        if (outP) *outP = bp;
+        if (outT) *outT = bt;
+        if (outM) *outM = bm;
     }
 
     // Morton-sort m device queries; fills s_qx/s_qy/s_qz and s_v0 (sorted -> caller index).
@@ -662,7 +965,7 @@ struct lvrhip_surface {
         s_v1.reserve(std::max<int64_t>(m, 1));
         s_hist.reserve((size_t)RS_RADIX * ceil_div(std::max<int64_t>(m, 1), RS_TILE));
         const unsigned gb = (unsigned)ceil_div(m, 256);
-        morton_queries_kernel<<<gb, 256, 0, st>>>(d_q, m, frame, s_k0.p, s_v0.p);
+        morton_queries_kernel<<<gb, 256, 0, st>>>(d_q, m, frames.p, s_k0.p, s_v0.p);
         LVRHIP_LAUNCH_CHECK();
         radix_sort_pairs(s_k0.p, s_v0.p, s_k1.p, s_v1.p, m, s_hist.p, st);
         s_qx.reserve(m);
@@ -672,27 +975,28 @@ struct lvrhip_surface {
         LVRHIP_LAUNCH_CHECK();
     }
 
+    void require_one_scan(const char* what) const
+    {
+        if (nscans > 1) throw Error(LVRHIP_EINVAL, std::string(what) + " needs a single-scan handle");
+    }
+
     void distances_device(const float* d_q, int64_t m, float voxel, float* d_proj, float* d_eucl, uint8_t* d_inv)
     {
         if (!has_normals) throw Error(LVRHIP_ESTATE, "distances called before calculate_normals");
+        require_one_scan("distances");
         if (m == 0) return;
         if (n == 0) throw Error(LVRHIP_ESTATE, "distances on an empty point cloud");
-        const int t0 = (int)std::min<int64_t>(kd, n);
+        drop_graph();
+        const int t0 = clamp_k(kd);
         s_k1.reserve(std::max<size_t>((size_t)m * t0, (size_t)m));
         s_spos.reserve((size_t)m * t0);
         s_ord.reserve((size_t)m * t0);
         const int tsort = prof.start(LVRHIP_T_SDF_SORT, st);
         sort_queries(d_q, m);
         prof.stop(tsort, st);
-        TileParams prm{};
-        prm.t0 = t0;
-        prm.t1 = 0;
-        prm.tiles_per_wave = tiles_per_wave;
-        prm.r2_init = r2_init(t0);
-        prm.r2_margin = r2_margin;
-        prm.err = err.p;
+        TileParams prm = params(t0, 0, 2);
         prm.stats = kstats_ptr(2);
-        QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m};
+        QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
         auto mk = [&](auto tag) {
             constexpr int P = decltype(tag)::value;
             SinkSdf<P> s{};
@@ -711,7 +1015,7 @@ struct lvrhip_surface {
             s.out_proj = d_proj;
             s.out_eucl = d_eucl;
             s.out_invalid = d_inv;
-            s.npts = (uint32_t)n;
+            s.npts = (uint32_t)npad;
             return s;
         };
         const int tok = prof.start(LVRHIP_T_SDF_KERNEL, st);
@@ -724,21 +1028,17 @@ struct lvrhip_surface {
 
     void knn_device(const float* d_q, int64_t m, int k, int32_t* d_idx, float* d_d2)
     {
+        require_one_scan("kSearch");
         if (m == 0) return;
+        drop_graph();
         const int t0 = (int)std::min<int64_t>(k, n);
         if (t0 > 0) {
             sort_queries(d_q, m);
             DBuf<uint64_t> keys;
             keys.reserve((size_t)m * k);
-            TileParams prm{};
-            prm.t0 = t0;
-            prm.t1 = 0;
-            prm.tiles_per_wave = tiles_per_wave;
-            prm.r2_init = r2_init(t0);
-            prm.r2_margin = r2_margin;
-            prm.err = err.p;
+            TileParams prm = params(t0, 0, 3);
             prm.stats = kstats_ptr(3);
-            QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m};
+            QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
                 SinkList<P> s{};
@@ -785,26 +1085,45 @@ int guarded(F&& f)
         if (!(cond)) throw Error(LVRHIP_EINVAL, std::string(msg));   \
     } while (0)
 
+// the same check in a function that returns the status itself
+#define REQUIRE_C(cond, msg)              \
+    do {                                  \
+        if (!(cond)) {                    \
+            g_last_error = (msg);         \
+            return LVRHIP_EINVAL;         \
+        }                                 \
+    } while (0)
+
 }  // namespace
 
 extern "C" {
 
 const char* lvrhip_last_error(void) { return g_last_error.c_str(); }
 
-int lvrhip_create_device(const float* d_xyz, size_t n, int device, void* stream, lvrhip_t* out)
+int lvrhip_create_scans_device(const float* d_xyz, const int64_t* scan_offsets, int nscans, int device, void* stream,
+                               lvrhip_t* out)
 {
     return guarded([&] {
         REQUIRE(out != nullptr, "out handle is NULL");
-        REQUIRE(n < (size_t)0xFFFFFFFFu - 64, "more than 2^32-65 points");
+        REQUIRE(scan_offsets != nullptr, "scan offsets pointer is NULL");
+        REQUIRE(nscans >= 1 && nscans <= 65535, "number of scans must be in [1, 65535]");
+        REQUIRE(scan_offsets[0] == 0, "scan offsets must start at 0");
+        int64_t padded = 0;
+        for (int s = 0; s < nscans; s++) {
+            REQUIRE(scan_offsets[s + 1] >= scan_offsets[s], "scan offsets must be non-decreasing");
+            padded += (scan_offsets[s + 1] - scan_offsets[s] + LEAF - 1) / LEAF * LEAF;
+        }
+        const int64_t n = scan_offsets[nscans];
+        REQUIRE(padded < (int64_t)0xFFFFFFFFu - 64, "more than 2^32-65 points (with padding)");
         REQUIRE(n == 0 || d_xyz != nullptr, "points pointer is NULL");
         DeviceGuard dg(device);
         auto* h = new lvrhip_surface;
         h->device = device;
         h->st = (hipStream_t)stream;
-        h->n = (int64_t)n;
         try {
-            h->err.reserve(1);
-            h->build(d_xyz);
+            h->init(d_xyz, scan_offsets, nscans);
+            h->build();
+            LVRHIP_CHECK(hipStreamSynchronize(h->st));
         } catch (...) {
             delete h;
             throw;
@@ -813,19 +1132,44 @@ int lvrhip_create_device(const float* d_xyz, size_t n, int device, void* stream,
     });
 }
 
-int lvrhip_create(const float* xyz, size_t n, int device, lvrhip_t* out)
+int lvrhip_create_device(const float* d_xyz, size_t n, int device, void* stream, lvrhip_t* out)
+{
+    REQUIRE_C(n < (size_t)0xFFFFFFFFu - 64, "more than 2^32-65 points");
+    const int64_t off[2] = {0, (int64_t)n};
+    return lvrhip_create_scans_device(d_xyz, off, 1, device, stream, out);
+}
+
+int lvrhip_create_scans(const float* xyz, const int64_t* scan_offsets, int nscans, int device, lvrhip_t* out)
 {
     return guarded([&] {
         REQUIRE(out != nullptr, "out handle is NULL");
+        REQUIRE(scan_offsets != nullptr && nscans >= 1, "scan table missing");
+        const int64_t n = scan_offsets[nscans];
+        REQUIRE(n >= 0, "negative point count");
         REQUIRE(n == 0 || xyz != nullptr, "points pointer is NULL");
         DeviceGuard dg(device);
-        DBuf<float> d;
-        d.reserve(std::max<size_t>(3 * n, 1));
-        if (n) LVRHIP_CHECK(hipMemcpy(d.p, xyz, 3 * n * sizeof(float), hipMemcpyHostToDevice));
-        int rc = lvrhip_create_device(d.p, n, device, nullptr, out);
-        if (rc != LVRHIP_OK) throw Error(rc, g_last_error);
-        LVRHIP_CHECK(hipDeviceSynchronize());
+        auto own = new DBuf<float>;
+        try {
+            own->reserve(std::max<size_t>(3 * (size_t)n, 1));
+            if (n) LVRHIP_CHECK(hipMemcpy(own->p, xyz, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+            int rc = lvrhip_create_scans_device(own->p, scan_offsets, nscans, device, nullptr, out);
+            if (rc != LVRHIP_OK) throw Error(rc, g_last_error);
+            // the handle keeps the device copy as the source of later rebuilds
+            std::swap((*out)->own_src.p, own->p);
+            std::swap((*out)->own_src.cap, own->cap);
+        } catch (...) {
+            delete own;
+            throw;
+        }
+        delete own;
     });
+}
+
+int lvrhip_create(const float* xyz, size_t n, int device, lvrhip_t* out)
+{
+    REQUIRE_C(n < (size_t)0xFFFFFFFFu - 64, "more than 2^32-65 points");
+    const int64_t off[2] = {0, (int64_t)n};
+    return lvrhip_create_scans(xyz, off, 1, device, out);
 }
 
 int lvrhip_destroy(lvrhip_t h)
@@ -859,6 +1203,7 @@ int lvrhip_set_flip_point(lvrhip_t h, float x, float y, float z)
         h->flip[0] = x;
         h->flip[1] = y;
         h->flip[2] = z;
+        h->scan_flips = false;
     });
 }
 
@@ -905,16 +1250,18 @@ int lvrhip_set_normals(lvrhip_t h, const float* normals)
         REQUIRE(h->n == 0 || normals, "normals pointer is NULL");
         DeviceGuard dg(h->device);
         const int64_t N = h->n;
+        h->drop_graph();
         h->ki_valid = false;
         h->has_normals = false;
         if (N > 0) {
-            h->nx.reserve(N);
-            h->ny.reserve(N);
-            h->nz.reserve(N);
+            const int64_t R = h->npad;
+            h->nx.reserve(R);
+            h->ny.reserve(R);
+            h->nz.reserve(R);
             DBuf<float> d;
             d.reserve(3 * (size_t)N);
             LVRHIP_CHECK(hipMemcpyAsync(d.p, normals, 3 * (size_t)N * sizeof(float), hipMemcpyHostToDevice, h->st));
-            gather_normals_kernel<<<(unsigned)ceil_div(N, 256), 256, 0, h->st>>>(d.p, h->pidx.p, N, h->nx.p, h->ny.p,
+            gather_normals_kernel<<<(unsigned)ceil_div(R, 256), 256, 0, h->st>>>(d.p, h->pidx.p, R, h->nx.p, h->ny.p,
                                                                                h->nz.p);
             LVRHIP_LAUNCH_CHECK();
             LVRHIP_CHECK(hipStreamSynchronize(h->st));
@@ -931,9 +1278,7 @@ int lvrhip_get_normals_device(lvrhip_t h, float* d_out)
         if (h->n == 0) return;
         REQUIRE(d_out, "output pointer is NULL");
         DeviceGuard dg(h->device);
-        scatter_normals_kernel<<<(unsigned)ceil_div(h->n, 256), 256, 0, h->st>>>(h->nx.p, h->ny.p, h->nz.p, h->pidx.p,
-                                                                                h->n, d_out);
-        LVRHIP_LAUNCH_CHECK();
+        h->scatter_normals(h->nx.p, h->ny.p, h->nz.p, d_out);
     });
 }
 
@@ -1092,6 +1437,92 @@ int lvrhip_get_timings(lvrhip_t h, double* ms, int64_t* launches, int reset)
                 h->prof.launches[i] = 0;
             }
         }
+    });
+}
+
+int lvrhip_num_scans(lvrhip_t h, int* nscans)
+{
+    return guarded([&] {
+        REQUIRE(h && nscans, "NULL argument");
+        *nscans = h->nscans;
+    });
+}
+
+int lvrhip_set_scan_flip_points(lvrhip_t h, const float* xyz)
+{
+    return guarded([&] {
+        REQUIRE(h && xyz, "NULL argument");
+        DeviceGuard dg(h->device);
+        h->drop_graph();
+        h->d_flips.reserve(3 * (size_t)h->nscans);
+        LVRHIP_CHECK(
+            hipMemcpyAsync(h->d_flips.p, xyz, 3 * (size_t)h->nscans * sizeof(float), hipMemcpyHostToDevice, h->st));
+        LVRHIP_CHECK(hipStreamSynchronize(h->st));
+        h->scan_flips = h->nscans > 1;
+        if (h->nscans == 1) {
+            h->flip[0] = xyz[0];
+            h->flip[1] = xyz[1];
+            h->flip[2] = xyz[2];
+        }
+    });
+}
+
+int lvrhip_rebuild(lvrhip_t h)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        DeviceGuard dg(h->device);
+        h->has_normals = false;
+        h->ki_valid = false;
+        h->build();
+        LVRHIP_CHECK(hipStreamSynchronize(h->st));
+    });
+}
+
+int lvrhip_graph_capture(lvrhip_t h, float* d_out_normals, int rebuild)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        REQUIRE(d_out_normals, "output pointer is NULL");
+        DeviceGuard dg(h->device);
+        h->graph_capture(d_out_normals, rebuild != 0);
+    });
+}
+
+int lvrhip_graph_launch(lvrhip_t h)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        DeviceGuard dg(h->device);
+        h->graph_launch();
+    });
+}
+
+int lvrhip_synchronize(lvrhip_t h)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        DeviceGuard dg(h->device);
+        h->synchronize();
+    });
+}
+
+int lvrhip_autotune(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float* r2_margin)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        DeviceGuard dg(h->device);
+        h->autotune(lanes_per_query, tiles_per_wave, r2_margin);
+    });
+}
+
+int lvrhip_get_tuning(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float* r2_margin)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        if (lanes_per_query) *lanes_per_query = h->lanes_per_query;
+        if (tiles_per_wave) *tiles_per_wave = h->tiles_per_wave;
+        if (r2_margin) *r2_margin = h->r2_margin;
     });
 }
 

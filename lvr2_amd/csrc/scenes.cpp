@@ -189,16 +189,19 @@ int scene_plane_boxes_sphere(uint64_t seed, int64_t n, double sigma, float* out)
     return 0;
 }
 
-// Terrestrial-scan-like cloud: rays from a scanner at the origin on a uniform
-// (azimuth 0..360, elevation -60..+90 deg) grid of n_az x n_el cell centres into a
-// 20 x 15 x 4 m room with n_obj random boxes / vertical cylinders standing on the floor;
-// radial N(0, sigma^2) noise.  Writes n_az*n_el points.
-int scene_room_scan(uint64_t seed, int n_az, int n_el, int n_obj, double sigma, float* out)
-{
-    Rng g(seed, 0x5EEDull);
-    const Box room{{-8.0, -6.0, -1.5}, {12.0, 9.0, 2.5}};
+struct Room {
+    Box walls;
     std::vector<Box> boxes;
     std::vector<Cyl> cyls;
+};
+
+// A 20 x 15 x 4 m room with n_obj random boxes / vertical cylinders standing on the floor,
+// keeping 0.6 m around the origin free.
+static Room make_room(uint64_t seed, int n_obj)
+{
+    Rng g(seed, 0x5EEDull);
+    Room rm{Box{{-8.0, -6.0, -1.5}, {12.0, 9.0, 2.5}}, {}, {}};
+    const Box& room = rm.walls;
     for (int i = 0; i < n_obj; i++) {
         bool is_box = g.uniform() < 0.5;
         double h = g.uniform(0.3, 2.5);
@@ -208,12 +211,19 @@ int scene_room_scan(uint64_t seed, int n_az, int n_el, int n_obj, double sigma, 
             double ext = is_box ? 0.5 * std::sqrt(sx * sx + sy * sy) : r;
             if (std::sqrt(cx * cx + cy * cy) < ext + 0.6) continue;  // keep the scanner free
             if (is_box)
-                boxes.push_back(Box{{cx - sx / 2, cy - sy / 2, room.lo[2]}, {cx + sx / 2, cy + sy / 2, room.lo[2] + h}});
+                rm.boxes.push_back(
+                    Box{{cx - sx / 2, cy - sy / 2, room.lo[2]}, {cx + sx / 2, cy + sy / 2, room.lo[2] + h}});
             else
-                cyls.push_back(Cyl{cx, cy, r, room.lo[2], room.lo[2] + h});
+                rm.cyls.push_back(Cyl{cx, cy, r, room.lo[2], room.lo[2] + h});
             break;
         }
     }
+    return rm;
+}
+
+static void cast_room(const Room& rm, uint64_t noise_seed, int n_az, int n_el, double sigma, const double o[3],
+                      float* out)
+{
     const int64_t total = (int64_t)n_az * n_el;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < total; i++) {
@@ -221,14 +231,56 @@ int scene_room_scan(uint64_t seed, int n_az, int n_el, int n_obj, double sigma, 
         double az = (ia + 0.5) * (2 * M_PI / n_az);
         double el = (-60.0 + (ie + 0.5) * (150.0 / n_el)) * (M_PI / 180.0);
         double d[3] = {std::cos(el) * std::cos(az), std::cos(el) * std::sin(az), std::sin(el)};
-        const double o[3] = {0, 0, 0};
-        double t = hit_room(o, d, room);
-        for (auto& b : boxes) t = std::min(t, hit_box(o, d, b));
-        for (auto& c : cyls) t = std::min(t, hit_cyl(o, d, c));
-        Rng r(seed, (uint64_t)i);
+        double t = hit_room(o, d, rm.walls);
+        for (auto& b : rm.boxes) t = std::min(t, hit_box(o, d, b));
+        for (auto& c : rm.cyls) t = std::min(t, hit_cyl(o, d, c));
+        Rng r(noise_seed, (uint64_t)i);
         t += sigma * r.gauss();
-        for (int j = 0; j < 3; j++) out[3 * i + j] = (float)(d[j] * t);
+        for (int j = 0; j < 3; j++) out[3 * i + j] = (float)(o[j] + d[j] * t);
     }
+}
+
+// Terrestrial-scan-like cloud: rays from a scanner at the origin on a uniform
+// (azimuth 0..360, elevation -60..+90 deg) grid of n_az x n_el cell centres into the room;
+// radial N(0, sigma^2) noise.  Writes n_az*n_el points.
+int scene_room_scan(uint64_t seed, int n_az, int n_el, int n_obj, double sigma, float* out)
+{
+    const Room rm = make_room(seed, n_obj);
+    const double o[3] = {0, 0, 0};
+    cast_room(rm, seed, n_az, n_el, sigma, o, out);
+    return 0;
+}
+
+// A scanner position in the room of `seed`: uniform over the floor area 0.5 m away from the
+// walls and every object, height U[-0.5, 0.5] m (pose_seed picks it).
+int scene_room_pose(uint64_t seed, int n_obj, uint64_t pose_seed, double* out)
+{
+    const Room rm = make_room(seed, n_obj);
+    Rng g(pose_seed, 0x9053ull);
+    for (int tries = 0; tries < 100000; tries++) {
+        const double x = g.uniform(rm.walls.lo[0] + 0.5, rm.walls.hi[0] - 0.5);
+        const double y = g.uniform(rm.walls.lo[1] + 0.5, rm.walls.hi[1] - 0.5);
+        const double z = g.uniform(-0.5, 0.5);
+        bool free = true;
+        for (auto& b : rm.boxes)
+            if (x > b.lo[0] - 0.5 && x < b.hi[0] + 0.5 && y > b.lo[1] - 0.5 && y < b.hi[1] + 0.5) free = false;
+        for (auto& c : rm.cyls)
+            if (std::hypot(x - c.cx, y - c.cy) < c.r + 0.5) free = false;
+        if (!free) continue;
+        out[0] = x;
+        out[1] = y;
+        out[2] = z;
+        return 0;
+    }
+    return 1;
+}
+
+// The room scanned from `pos` (as scene_room_scan), noise drawn from noise_seed.
+int scene_room_scan_at(uint64_t seed, int n_az, int n_el, int n_obj, double sigma, const double* pos,
+                       uint64_t noise_seed, float* out)
+{
+    const Room rm = make_room(seed, n_obj);
+    cast_room(rm, noise_seed, n_az, n_el, sigma, pos, out);
     return 0;
 }
 

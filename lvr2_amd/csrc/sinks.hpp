@@ -116,16 +116,18 @@ __device__ inline void plane_from_moments(int method, int kcount, float qx, floa
         n2 = d1x * d2y - d1y * d2x;
         normalize3(n0, n1, n2);
     } else {
-        // calcPlaneIPCAExact (tcc:609-673): covariance of {q} + k-set, smallest eigenvector
-        const float cntf = (float)(kcount + 1);
-        const float mx = sx / cntf, my = sy / cntf, mz = sz / cntf;
-        float C[6];
-        C[0] = sxx / cntf - mx * mx;
-        C[1] = sxy / cntf - mx * my;
-        C[2] = sxz / cntf - mx * mz;
-        C[3] = syy / cntf - my * my;
-        C[4] = syz / cntf - my * mz;
-        C[5] = szz / cntf - mz * mz;
+        // calcPlaneIPCAExact (tcc:609-673): covariance of {q} + k-set, smallest eigenvector.
+        // The covariance is formed in fp64 from the fp32 moments: its cancellation
+        // (S2 / n - m m^T) would otherwise add eps32 * |m|^2 / l_max to the matrix.
+        const double cnt = (double)(kcount + 1);
+        const double mx = sx / cnt, my = sy / cnt, mz = sz / cnt;
+        double C[6];
+        C[0] = sxx / cnt - mx * mx;
+        C[1] = sxy / cnt - mx * my;
+        C[2] = sxz / cnt - mx * mz;
+        C[3] = syy / cnt - my * my;
+        C[4] = syz / cnt - my * mz;
+        C[5] = szz / cnt - mz * mz;
         smallest_eigvec(C, n0, n1, n2);
     }
 }
@@ -224,24 +226,47 @@ struct SinkNormals {
 
 // Plane fit + flip (AdaptiveKSearchSurface.tcc:225-274) for the queries of one adaptive round
 // whose neighbourhood passed boundingBoxOK (or all of them in the last round).
-__global__ __launch_bounds__(256) void plane_fit_kernel(const uint32_t* __restrict__ qsel, int64_t nq,
-                                                        const uint32_t* __restrict__ retry,
-                                                        const float* __restrict__ mom, const float* __restrict__ px,
-                                                        const float* __restrict__ py, const float* __restrict__ pz,
-                                                        int method, int kcount, float fx, float fy, float fz,
-                                                        float* __restrict__ nx, float* __restrict__ ny,
-                                                        float* __restrict__ nz, uint8_t* __restrict__ kround, int round)
+struct PlaneFitParams {
+    const uint32_t* qsel;     // query slot -> sorted position (NULL: identity)
+    int64_t nq;               // query slots (upper bound when nq_dev is set)
+    const uint32_t* nq_dev;   // optional device-side count of query slots
+    const uint32_t* retry;    // per slot: 1 = bbox test failed, plane fit deferred (NULL: last round)
+    const float* mom;         // [spos * 9] moments of the k-set about the point
+    const float* px;
+    const float* py;
+    const float* pz;
+    int method;
+    int kcount;               // |k-set| of this round
+    float fx, fy, fz;         // flip point (one scan)
+    const float* flips;       // per scan flip points (x, y, z), overrides fx/fy/fz
+    const uint16_t* scan_of;  // per sorted position: scan
+    float* nx;
+    float* ny;
+    float* nz;
+    uint8_t* kround;
+    int round;
+};
+
+__global__ __launch_bounds__(256) void plane_fit_kernel(PlaneFitParams a)
 {
     const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nq = a.nq_dev ? (int64_t)*a.nq_dev : a.nq;
     if (qi >= nq) return;
-    if (retry && retry[qi]) return;
-    const uint32_t qpos = qsel ? qsel[qi] : (uint32_t)qi;
-    const float qx = px[qpos], qy = py[qpos], qz = pz[qpos];
+    if (a.retry && a.retry[qi]) return;
+    const uint32_t qpos = a.qsel ? a.qsel[qi] : (uint32_t)qi;
+    const float qx = a.px[qpos], qy = a.py[qpos], qz = a.pz[qpos];
     float m[9];
 #pragma unroll
-    for (int i = 0; i < 9; i++) m[i] = mom[(size_t)qpos * 9 + i];
+    for (int i = 0; i < 9; i++) m[i] = a.mom[(size_t)qpos * 9 + i];
     float n0, n1, n2;
-    plane_from_moments(method, kcount, qx, qy, qz, m, n0, n1, n2);
+    plane_from_moments(a.method, a.kcount, qx, qy, qz, m, n0, n1, n2);
+    float fx = a.fx, fy = a.fy, fz = a.fz;
+    if (a.flips) {
+        const int sc = a.scan_of ? (int)a.scan_of[qpos] : 0;
+        fx = a.flips[3 * sc];
+        fy = a.flips[3 * sc + 1];
+        fz = a.flips[3 * sc + 2];
+    }
     // flip towards the flip point (tcc:268-274); Normal::operator- renormalises
     const float ddx = fx - qx, ddy = fy - qy, ddz = fz - qz;
     if (n0 * ddx + n1 * ddy + n2 * ddz < 0.0f) {
@@ -250,10 +275,10 @@ __global__ __launch_bounds__(256) void plane_fit_kernel(const uint32_t* __restri
         n2 = -n2;
         normalize3(n0, n1, n2);
     }
-    nx[qpos] = n0;
-    ny[qpos] = n1;
-    nz[qpos] = n2;
-    if (kround) kround[qpos] = (uint8_t)round;
+    a.nx[qpos] = n0;
+    a.ny[qpos] = n1;
+    a.nz[qpos] = n2;
+    if (a.kround) a.kround[qpos] = (uint8_t)a.round;
 }
 
 template <int P>

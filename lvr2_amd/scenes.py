@@ -28,6 +28,9 @@ def _l():
         l.scene_plane_boxes_sphere.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_void_p]
         l.scene_room_scan.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]
         l.scene_sphere_torus.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_void_p]
+        l.scene_room_pose.argtypes = [C.c_uint64, C.c_int, C.c_uint64, C.c_void_p]
+        l.scene_room_scan_at.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p,
+                                         C.c_uint64, C.c_void_p]
         _lib = l
     return _lib
 
@@ -41,6 +44,22 @@ def plane_boxes_sphere(seed: int, n: int, sigma: float = 0.5) -> np.ndarray:
 def room_scan(seed: int, n_az: int, n_el: int, n_obj: int = 40, sigma: float = 0.002) -> np.ndarray:
     out = np.empty((n_az * n_el, 3), np.float32)
     _l().scene_room_scan(seed, n_az, n_el, n_obj, sigma, out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def room_pose(seed: int, pose_seed: int, n_obj: int = 40) -> np.ndarray:
+    out = np.empty(3, np.float64)
+    if _l().scene_room_pose(seed, n_obj, pose_seed, out.ctypes.data_as(C.c_void_p)) != 0:
+        raise RuntimeError("no free scanner position found")
+    return out
+
+
+def room_scan_at(seed: int, n_az: int, n_el: int, pos, noise_seed: int, n_obj: int = 40,
+                 sigma: float = 0.002) -> np.ndarray:
+    out = np.empty((n_az * n_el, 3), np.float32)
+    p = np.ascontiguousarray(pos, np.float64)
+    _l().scene_room_scan_at(seed, n_az, n_el, n_obj, sigma, p.ctypes.data_as(C.c_void_p), noise_seed,
+                            out.ctypes.data_as(C.c_void_p))
     return out
 
 
@@ -73,3 +92,32 @@ def room(n_az: int = 3200, n_el: int = 1250, seed: int = 2) -> Workload:
 
 def sphere_torus_1m(n: int = 1_000_000, seed: int = 5) -> Workload:
     return Workload("sphere_torus", sphere_torus(seed, n, 1.0, 0.01), 30, 0, 10, (0.0, 0.0, 0.0), 0.05)
+
+
+@dataclass
+class ScanProject:
+    name: str
+    points: np.ndarray  # all scans concatenated
+    sizes: np.ndarray   # points per scan
+    flips: np.ndarray   # scanner position per scan (the flip point)
+    kn: int
+    ki: int
+    kd: int
+
+
+def scan_project(nscans: int = 64, lo: int = 50_000, hi: int = 2_000_000, seed: int = 4, kn: int = 50,
+                 ki: int = 20, kd: int = 10) -> ScanProject:
+    """Many scans of one room from random scanner positions, sizes log-uniform in [lo, hi]
+    (each an n_az x n_el ray grid with the room scan's 2.56 : 1 aspect)."""
+    rng = np.random.default_rng(seed)
+    target = np.exp(rng.uniform(np.log(lo), np.log(hi), nscans))
+    pts, sizes, flips = [], [], []
+    for s_, t in enumerate(target):
+        n_el = max(4, int(round(np.sqrt(t / 2.56))))
+        n_az = max(4, int(round(t / n_el)))
+        pos = room_pose(seed, 1000 + s_)
+        pts.append(room_scan_at(seed, n_az, n_el, pos, 7000 + s_))
+        sizes.append(n_az * n_el)
+        flips.append(pos)
+    return ScanProject(f"scan_project_{nscans}", np.concatenate(pts), np.asarray(sizes, np.int64),
+                       np.asarray(flips, np.float32), kn, ki, kd)

@@ -28,30 +28,54 @@ def _is_device_tensor(a) -> bool:
     return hasattr(a, "is_cuda") and bool(getattr(a, "is_cuda"))
 
 
-class HipSurface:
-    """Drop-in for ``CudaSurface(floatArr& points, size_t num_points, int device)``."""
+def _scan_table(sizes) -> np.ndarray:
+    sz = np.asarray(sizes, dtype=np.int64).ravel()
+    if sz.size < 1 or (sz < 0).any():
+        raise ValueError("scan sizes must be a non-empty list of non-negative counts")
+    return np.concatenate([[0], np.cumsum(sz)]).astype(np.int64)
 
-    def __init__(self, points, device: int = 0, stream=None):
+
+class HipSurface:
+    """Drop-in for ``CudaSurface(floatArr& points, size_t num_points, int device)``.
+
+    ``scan_sizes`` makes it the batched form: the points are the concatenation of scans of
+    those sizes, each searched on its own (estimateProjectNormals, ScanProjectUtils.cpp:361-410).
+    With a device tensor the tensor must stay alive for ``rebuild`` and graph replays; the
+    surface keeps a reference to it.
+    """
+
+    def __init__(self, points, device: int = 0, stream=None, scan_sizes=None):
         self._h = C.c_void_p()
         self._keep = None
         if _is_device_tensor(points):
             import torch
 
-            t = points.detach().to(torch.float32).contiguous()
+            t = points.detach()
+            if t.dtype != torch.float32 or not t.is_contiguous():
+                t = t.to(torch.float32).contiguous()
             if t.ndim != 2 or t.shape[1] != 3:
                 raise ValueError("points must have shape (N, 3)")
             self._n = int(t.shape[0])
+            self._keep = t
             device = t.device.index if t.device.index is not None else device
             st = stream if stream is not None else torch.cuda.current_stream(t.device).cuda_stream
-            check(lib().lvrhip_create_device(C.c_void_p(t.data_ptr()), self._n, device, C.c_void_p(st),
-                                             C.byref(self._h)))
+            off = _scan_table([self._n] if scan_sizes is None else scan_sizes)
+            if off[-1] != self._n:
+                raise ValueError("scan sizes must add up to the number of points")
+            check(lib().lvrhip_create_scans_device(C.c_void_p(t.data_ptr()), off.ctypes.data_as(C.c_void_p),
+                                                   len(off) - 1, device, C.c_void_p(st), C.byref(self._h)))
             torch.cuda.current_stream(t.device).synchronize()
         else:
             p = _as_host_points(points)
             self._n = int(p.shape[0])
-            check(lib().lvrhip_create(p.ctypes.data_as(C.c_void_p), self._n, device, C.byref(self._h)))
+            off = _scan_table([self._n] if scan_sizes is None else scan_sizes)
+            if off[-1] != self._n:
+                raise ValueError("scan sizes must add up to the number of points")
+            check(lib().lvrhip_create_scans(p.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p),
+                                            len(off) - 1, device, C.byref(self._h)))
             if stream is not None:
                 self.setStream(stream)
+        self.nscans = len(off) - 1
         self.device = device
         self._kn, self._ki, self._kd = 10, 10, 5
 
@@ -92,6 +116,13 @@ class HipSurface:
 
     def setFlippoint(self, x: float, y: float, z: float) -> None:
         check(lib().lvrhip_set_flip_point(self._h, float(x), float(y), float(z)))
+
+    def setScanFlippoints(self, flips) -> None:
+        """One flip point per scan, (nscans, 3)."""
+        f = _as_host_points(flips)
+        if f.shape[0] != self.nscans:
+            raise ValueError("one flip point per scan")
+        check(lib().lvrhip_set_scan_flip_points(self._h, f.ctypes.data_as(C.c_void_p)))
 
     def setMethod(self, method) -> None:
         """"PCA" (calcPlaneIPCAExact), "ITERATIVE" (calcPlaneIterative), "LS" (calcPlane),
@@ -149,6 +180,35 @@ class HipSurface:
 
     def setTuning(self, lanes_per_query: int = 0, tiles_per_wave: int = 0, r2_margin: float = 0.0) -> None:
         check(lib().lvrhip_set_tuning(self._h, int(lanes_per_query), int(tiles_per_wave), float(r2_margin)))
+
+    def autotune(self) -> dict:
+        """Pick lanes per query / tiles per wave / radius margin by timing a sample."""
+        p, t, m = C.c_int(), C.c_int(), C.c_float()
+        check(lib().lvrhip_autotune(self._h, C.byref(p), C.byref(t), C.byref(m)))
+        return {"lanes_per_query": p.value, "tiles_per_wave": t.value, "r2_margin": m.value}
+
+    def tuning(self) -> dict:
+        p, t, m = C.c_int(), C.c_int(), C.c_float()
+        check(lib().lvrhip_get_tuning(self._h, C.byref(p), C.byref(t), C.byref(m)))
+        return {"lanes_per_query": p.value, "tiles_per_wave": t.value, "r2_margin": m.value}
+
+    # -- rebuild and graphs -----------------------------------------------------------------------
+    def rebuild(self) -> None:
+        """Rebuild the LBVH from the point buffer (after the points were changed in place)."""
+        check(lib().lvrhip_rebuild(self._h))
+
+    def captureGraph(self, out, rebuild: bool = True) -> None:
+        """Capture [build +] normals + interpolation + copy to ``out`` (device (N, 3) float32)."""
+        if not _is_device_tensor(out) or tuple(out.shape) != (self._n, 3) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous device tensor of shape (N, 3)")
+        self._graph_out = out
+        check(lib().lvrhip_graph_capture(self._h, C.c_void_p(out.data_ptr()), int(bool(rebuild))))
+
+    def launchGraph(self) -> None:
+        check(lib().lvrhip_graph_launch(self._h))
+
+    def synchronize(self) -> None:
+        check(lib().lvrhip_synchronize(self._h))
 
     def normalsStats(self) -> np.ndarray:
         """Points searched in each adaptive round (k = kn*2, kn*4, ..., kn*32)."""

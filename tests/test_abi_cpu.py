@@ -52,3 +52,24 @@ def test_create_without_gpu_raises_cleanly():
 
     with pytest.raises(LvrHipError):
         HipSurface(np.zeros((4, 3), np.float32))
+
+
+def test_scan_table_is_checked_without_gpu():
+    import numpy as np
+
+    from lvr2_amd import _lib
+
+    lib = _lib.lib()
+    out = C.c_void_p()
+    pts = np.zeros((10, 3), np.float32)
+    bad = [np.array([1, 10], np.int64),          # does not start at 0
+           np.array([0, 6, 4, 10], np.int64)]    # decreasing
+    for off in bad:
+        rc = lib.lvrhip_create_scans_device(pts.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p),
+                                            len(off) - 1, 0, None, C.byref(out))
+        assert rc == 1, lib.lvrhip_last_error()
+    off = np.array([0, 10], np.int64)
+    assert lib.lvrhip_create_scans_device(pts.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p), 0, 0,
+                                          None, C.byref(out)) == 1
+    assert lib.lvrhip_graph_launch(None) == 1
+    assert lib.lvrhip_autotune(None, None, None, None) == 1

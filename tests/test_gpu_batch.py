@@ -1,0 +1,213 @@
+"""GPU parity of the batched form (many scans in one handle), the hipGraph form and the
+autotuner.
+
+Bars (written into each test):
+  batched normals, per scan, vs the FP64 oracle on that scan alone    <= 1e-5 rad (same
+      exclusions as the single-scan tests: FP64 eigen gap < 1e-2 (<= 1e-4 down to 1e-3),
+      flip-ambiguous points)
+  batched adaptive round counts vs the per-scan oracle                 exact
+  batched interpolation given the oracle's normals                     bit-exact
+  graph replays vs the eager pass                                      bit-exact, every replay
+"""
+import numpy as np
+import pytest
+
+from conftest import angles, axis_angles
+
+pytestmark = pytest.mark.gpu
+
+NORMAL_TOL = 1e-5
+GAP_MIN = 1e-2   # below: the near-degenerate bucket, where fp32 moments bound the error by ~eps32 / gap
+NEAR_GAP = 1e-3
+NEAR_TOL = 1e-4  # bar for FP64 eigen gaps in [NEAR_GAP, GAP_MIN)
+
+
+def _project(grids, seed=11):
+    from lvr2_amd import scenes
+
+    pts, flips = [], []
+    for i, (n_az, n_el) in enumerate(grids):
+        if n_az * n_el == 0:
+            pts.append(np.zeros((0, 3), np.float32))
+            flips.append(np.zeros(3))
+            continue
+        pos = scenes.room_pose(seed, 100 + i)
+        pts.append(scenes.room_scan_at(seed, n_az, n_el, pos, 500 + i))
+        flips.append(pos)
+    return pts, np.asarray(flips, np.float32)
+
+
+# ray grids (azimuth x elevation) of the scans: an empty scan, sizes that are no multiple
+# of 16, a smallest scan of 348 points (>= 32 kn for kn = 10)
+GRIDS = [(250, 96), (0, 0), (89, 36), (320, 125), (29, 12)]
+SIZES = [a * e for a, e in GRIDS]
+
+
+@pytest.fixture(scope="module")
+def project(oracle_mod):
+    pts, flips = _project(GRIDS)
+    kn, ki = 10, 10
+    per_scan = []
+    for p, f in zip(pts, flips):
+        if len(p) == 0:
+            per_scan.append(None)
+            continue
+        o = oracle_mod.Surface(p)
+        n32, n64, k, gap = o.normals(kn, 3, tuple(f), gap=True)
+        per_scan.append((o, n32, n64, k, gap))
+    return pts, flips, kn, ki, per_scan
+
+
+def _batch(pts, flips, kn, ki):
+    from lvr2_amd import HipSurface
+
+    s = HipSurface(np.concatenate(pts), scan_sizes=[len(p) for p in pts])
+    s.setKn(kn)
+    s.setKi(ki)
+    s.setScanFlippoints(flips)
+    return s
+
+
+def test_batched_normals_match_each_scan(project):
+    pts, flips, kn, ki, per_scan = project
+    s = _batch(pts, flips, kn, 0)
+    assert s.nscans == len(SIZES)
+    s.calculateNormals()
+    g = s.getNormals()
+    counts = np.zeros(5, np.int64)
+    off = 0
+    for p, f, ref in zip(pts, flips, per_scan):
+        gs = g[off: off + len(p)]
+        off += len(p)
+        if ref is None:
+            continue
+        o, n32, n64, k, gap = ref
+        counts += [np.sum(k >= kn * 2 ** (r + 1)) for r in range(5)]
+        flipdir = f.astype(np.float64) - p.astype(np.float64)
+        amb = np.abs(np.sum(n64 * flipdir, 1)) < 1e-4 * np.linalg.norm(flipdir, axis=1)
+        good = (gap >= GAP_MIN) & ~amb
+        a = angles(gs, n64)
+        assert np.max(a[good]) <= NORMAL_TOL, np.max(a[good])
+        near = (gap >= NEAR_GAP) & (gap < GAP_MIN) & ~amb
+        assert not near.any() or np.max(a[near]) <= NEAR_TOL
+        # oriented towards this scan's own scanner
+        assert np.mean(np.sum(gs.astype(np.float64) * flipdir, 1) >= 0) > 0.999
+    np.testing.assert_array_equal(s.normalsStats(), counts)
+
+
+def test_batched_interpolation_bitexact_given_normals(project):
+    pts, flips, kn, ki, per_scan = project
+    s = _batch(pts, flips, kn, ki)
+    given = np.concatenate([ref[1] if ref is not None else np.zeros((0, 3), np.float32) for ref in per_scan])
+    s.setNormals(given)
+    s.interpolateNormals()
+    g = s.getNormals()
+    off = 0
+    for p, ref in zip(pts, per_scan):
+        if ref is not None:
+            np.testing.assert_array_equal(g[off: off + len(p)], ref[0].interpolate(ref[1], ki))
+        off += len(p)
+    # ki > 2 kn takes the separate ki search, also per scan
+    s.setKi(3 * kn)
+    s.setNormals(given)
+    s.interpolateNormals()
+    g = s.getNormals()
+    off = 0
+    for p, ref in zip(pts, per_scan):
+        if ref is not None:
+            np.testing.assert_array_equal(g[off: off + len(p)], ref[0].interpolate(ref[1], 3 * kn))
+        off += len(p)
+
+
+def test_batched_rejects_small_scans_and_single_scan_calls(project):
+    from lvr2_amd import LvrHipError
+
+    pts, flips, kn, ki, _ = project
+    s = _batch(pts, flips, kn, ki)
+    s.setKn(11)  # 32 * 11 = 352 > the 348-point scan
+    with pytest.raises(LvrHipError) as e:
+        s.calculateNormals()
+    assert e.value.code == 1
+    with pytest.raises(LvrHipError) as e:
+        s.kSearch(pts[0][:10], 5)
+    assert e.value.code == 1
+
+
+def _graph_replays(s, n, replays=3):
+    import torch
+
+    s.calculateNormals()
+    s.interpolateNormals()
+    eager = s.getNormals()
+    out = torch.full((n, 3), float("nan"), dtype=torch.float32, device="cuda")
+    s.captureGraph(out, rebuild=True)
+    for r in range(replays):
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        s.launchGraph()
+        s.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), eager, err_msg=f"replay {r}")
+        # eager work of the program between replays
+        x = torch.randn(1 << 20, device="cuda")
+        (x * 2).sum().item()
+    return out, eager
+
+
+def test_graph_replays_single_scan():
+    from lvr2_amd import HipSurface, scenes
+
+    w = scenes.plain_small(60_000)
+    s = HipSurface(w.points)
+    s.setKn(w.kn)
+    s.setKi(w.ki)
+    s.setFlippoint(*w.flip)
+    _graph_replays(s, len(w.points))
+    st = s.normalsStats()
+    assert st[0] == len(w.points)
+
+
+def test_graph_replays_batched_and_moved_points(project):
+    import torch
+
+    from lvr2_amd import HipSurface
+
+    pts, flips, kn, ki, _ = project
+    allp = np.concatenate(pts)
+    sizes = [len(p) for p in pts]
+    t = torch.from_numpy(allp).cuda()
+    s = HipSurface(t, scan_sizes=sizes)
+    s.setKn(kn)
+    s.setKi(ki)
+    s.setScanFlippoints(flips)
+    out, eager = _graph_replays(s, len(allp))
+    # move the points in place: the replay rebuilds the hierarchy from the same buffer and
+    # matches a fresh handle on the moved points (same flip points) bit for bit
+    moved = allp + np.random.default_rng(3).normal(scale=1e-3, size=allp.shape).astype(np.float32)
+    t.copy_(torch.from_numpy(moved))
+    torch.cuda.synchronize()
+    s.launchGraph()
+    s.synchronize()
+    fresh = HipSurface(moved, scan_sizes=sizes)
+    fresh.setKn(kn)
+    fresh.setKi(ki)
+    fresh.setScanFlippoints(flips)
+    fresh.calculateNormals()
+    fresh.interpolateNormals()
+    np.testing.assert_array_equal(out.cpu().numpy(), fresh.getNormals())
+    assert not np.array_equal(fresh.getNormals(), eager)
+
+
+def test_autotune_keeps_results(project):
+    pts, flips, kn, ki, per_scan = project
+    s = _batch(pts, flips, kn, 0)
+    s.calculateNormals()
+    before = s.getNormals()
+    stats = s.normalsStats()
+    best = s.autotune()
+    assert best["lanes_per_query"] in (4, 8)
+    assert best["tiles_per_wave"] in (4, 8, 16)
+    assert best == s.tuning()
+    s.calculateNormals()
+    np.testing.assert_array_equal(s.normalsStats(), stats)
+    # other launch shapes sum the moments in another order: fp32 rounding differences only
+    assert np.quantile(axis_angles(before, s.getNormals()), 0.999) < 1e-5
