@@ -10,7 +10,8 @@ cell, all cell corners).
 A step = calculateNormals (adaptive kNN + PCA + flip) + interpolateNormals + distances at all
 query vertices, inputs resident in HBM.  value = points / (t_normals + t_interpolate) in
 Mpts/s; the SDF rate is reported beside it.  The roofline entry covers the dominant kernel
-(fused kNN + PCA, first adaptive round), timed with hipEvents on the library's stream.
+(the kNN tile kernel of the normals over its adaptive-round launches), timed with hipEvents
+on the library's stream.
 """
 from __future__ import annotations
 
@@ -120,7 +121,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--cpu-sample-q", type=int, default=300_000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel (y = x * 1) over MIB MiB three times: the known-size "
+                         "reference for HBM counter scaling")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        x = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 3 x (read {args.copy_calibration} MiB + write {args.copy_calibration} MiB)")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
               file=sys.stderr)
@@ -188,25 +202,36 @@ def main():
     mpts = n / ((t_norm + t_int) / 1e3) / 1e6
     sdf_mqps = m / (t_sdf / 1e3) / 1e6
 
-    # roofline of the dominant kernel: fused kNN + PCA, first adaptive round (k = 2 kn)
-    k1 = 2 * w.kn
-    bytes_per_pt = 12 + 12 * k1 + 12 + 4 * w.ki
+    # roofline of the dominant kernel: the kNN tile kernel of the normals (fused bounding-box
+    # test + moments), over its five adaptive-round launches.  Algorithmic bytes of a launch at
+    # k: per query point 12 (point) + 12 k (neighbours) + 12 (normal) + 4 ki (ki list, first
+    # round only), the per-unit figure of the fused kNN + PCA step.
+    lanes = s.tuning()["lanes_per_query"]
     ms_first, n_first = tm["normals_first"]
-    avg_ms = ms_first / max(n_first, 1)
-    achieved = n * bytes_per_pt / (avg_ms / 1e3) / 1e9
+    ms_retry, n_retry = tm["normals_retry"]
+    launches = n_first + n_retry
+    per_step = [int(x) for x in rounds]
+    bytes_step = 0
+    for r, nr in enumerate(per_step):
+        k_r = w.kn << (r + 1)
+        bytes_step += nr * (12 + 12 * k_r + 12) + (n * 4 * w.ki if r == 0 else 0)
+    launches_per_step = len(per_step)
+    avg_ms = (ms_first + ms_retry) / max(launches, 1)
+    achieved = bytes_step / launches_per_step / (avg_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
-        "kernel": "knn_tile_kernel<4, SinkNormals<4>> (first adaptive round)",
+        "kernel": f"knn_tile_kernel<{lanes}, SinkNormals<{lanes}>> (all adaptive rounds)",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
         "avg_launch_ms": round(avg_ms, 4),
-        "bytes_per_unit": bytes_per_pt,
-        "units_per_launch": n,
+        "bytes_per_launch": int(bytes_step / launches_per_step),
+        "units_per_launch": int(sum(per_step) / launches_per_step),
+        "sum_k_per_step": int(sum(nr * (w.kn << (r + 1)) for r, nr in enumerate(per_step))),
     }
-    traffic_file = os.path.join(ROOT, "profiles", "traffic_knn_pca_room_4m.json")
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_knn_normals_room_4m.json")
     if os.path.exists(traffic_file):
         try:
             roofline["traffic"] = json.load(open(traffic_file)).get("hbm_bytes_per_launch")

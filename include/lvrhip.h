@@ -176,8 +176,8 @@ int lvrhip_size(lvrhip_t h, size_t* n);
  * launches and the summed elapsed milliseconds (read after the call that launched them
  * returns).  Timers: */
 enum {
-    LVRHIP_T_NORMALS_FIRST = 0, /* fused kNN + plane fit kernel, first adaptive round */
-    LVRHIP_T_NORMALS_RETRY = 1, /* later adaptive rounds incl. their list compaction */
+    LVRHIP_T_NORMALS_FIRST = 0, /* kNN tile kernel of the normals, first adaptive round */
+    LVRHIP_T_NORMALS_RETRY = 1, /* the same kernel in the later adaptive rounds */
     LVRHIP_T_INTERPOLATE = 2,   /* interpolation kernel */
     LVRHIP_T_SDF_KERNEL = 3,    /* fused kNN + signed distance kernel */
     LVRHIP_T_SDF_SORT = 4,      /* Morton sort of the query vertices */

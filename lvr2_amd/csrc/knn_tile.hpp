@@ -109,7 +109,7 @@ struct WaveStats {
     {
         if (timing) c[i] += cycles() - t0;
     }
-    __device__ void flush(unsigned long long* dst)
+    __device__ __forceinline__ void flush(unsigned long long* dst)
     {
         if (!dst || lane_id() != 0) return;
 #pragma unroll
@@ -152,7 +152,7 @@ struct TileBox {
 // continuing the traversal held on the LDS stack (sp entries).  Stops when the stack is
 // empty or the list is full; returns the list length (wave-uniform).  sp = -1 on overflow.
 template <class Sm>
-__device__ int fill_leaves(Sm& sm, const BvhView& bv, const TileBox& tb, float r2, int& sp, int cap, WaveStats& ws)
+__device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const TileBox& tb, float r2, int& sp, int cap, WaveStats& ws)
 {
     const int lane = lane_id();
     int nl = 0;
@@ -210,7 +210,7 @@ __device__ int fill_leaves(Sm& sm, const BvhView& bv, const TileBox& tb, float r
 // Stage the points of leaves sm.tleaf[base, base + nl) in LDS; slot = j * LEAF + offset.
 // Slots past the end of the cloud get NaN coordinates (never in range).
 template <class Sm>
-__device__ void load_chunk(Sm& sm, const BvhView& bv, int base, int nl)
+__device__ __forceinline__ void load_chunk(Sm& sm, const BvhView& bv, int base, int nl)
 {
     constexpr int LPI = kWave / LEAF;             // leaves per iteration
     constexpr int IT = CHUNK_LEAVES / LPI;         // iterations for a full chunk
@@ -255,7 +255,7 @@ __device__ __forceinline__ uint32_t slot_spos(const Sm& sm, int base, int c)
 // sqrt(rq) of the query point, as chunk slots in sm.glist[g]; returns their count
 // (replicated over the group's lanes).  rq < 0 disables the query.
 template <int P, class Sm>
-__device__ int build_glist(Sm& sm, const BvhView& bv, int base, int nl, int g, int s, float qx, float qy, float qz,
+__device__ __forceinline__ int build_glist(Sm& sm, const BvhView& bv, int base, int nl, int g, int s, float qx, float qy, float qz,
                            float rq)
 {
     constexpr uint64_t GM = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
@@ -284,9 +284,11 @@ struct Cands {
 };
 
 // Calls f(nl, base) once per staged chunk of nl leaves starting at sm.tleaf[base].
+// The kernel's helpers are all inlined: a call would move the structs it passes by
+// reference (views, counters) into scratch memory.
 template <class Sm, class F>
-__device__ bool for_each_chunk(Sm& sm, const BvhView& bv, const TileBox& tb, float R2, int root, Cands& cs,
-                               WaveStats& ws, F&& f)
+__device__ __forceinline__ bool for_each_chunk(Sm& sm, const BvhView& bv, const TileBox& tb, float R2, int root,
+                                               Cands& cs, WaveStats& ws, F&& f)
 {
     if (cs.ntl >= 0) {
         if (cs.resident) {
@@ -405,7 +407,7 @@ __device__ __forceinline__ int vbin(float d2, float inv_w)
 }
 
 // Smallest non-negative float x with vbin(x, inv_w) >= b (b >= 1).
-__device__ float vbin_floor(int b, float inv_w)
+__device__ __forceinline__ float vbin_floor(int b, float inv_w)
 {
     float x = (float)b / inv_w;
     if (!(x >= 0.0f) || isinf(x)) x = 0.0f;
@@ -427,7 +429,9 @@ struct Target {
 };
 
 template <int P, class Sink>
-__global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView qv, TileParams prm, Sink sink)
+// LDS allows about two of these waves per SIMD: the register budget is set for that occupancy.
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) void knn_tile_kernel(
+    BvhView bv, QueryView qv, TileParams prm, Sink sink)
 {
     constexpr int Q = kWave / P;
     constexpr int BUF = 12;
@@ -616,7 +620,7 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
             }
             __syncthreads();
             const float rc = ok ? fmaxf(tg[0].dhi, prm.t1 > 0 ? tg[1].dhi : 0.0f) : -1.0f;
-            fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
+            auto collect = [&](int nl, int base) {
                 auto t0 = ws.tick();
                 const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rc);
                 ws.tock(KS_CY_GLIST, t0);
@@ -645,7 +649,8 @@ __global__ __launch_bounds__(kWave) void knn_tile_kernel(BvhView bv, QueryView q
                     }
                 });
                 ws.tock(KS_CY_COLLECT, t0);
-            });
+            };
+            fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, collect);
             if (!fine) {
                 if (lane == 0) atomicMax(prm.err, 5);
                 return;

@@ -270,14 +270,14 @@ struct SinkKiOnly {
     uint32_t* out;
     int kw;
     uint32_t qpos_;
-    __device__ void begin(int, int, int64_t, uint32_t qpos, float, float, float) { qpos_ = qpos; }
-    __device__ void accept(int tau, int g, int, float, float, float, uint64_t, uint32_t sp, uint32_t* cnt)
+    __device__ __forceinline__ void begin(int, int, int64_t, uint32_t qpos, float, float, float) { qpos_ = qpos; }
+    __device__ __forceinline__ void accept(int tau, int g, int, float, float, float, uint64_t, uint32_t sp, uint32_t* cnt)
     {
         if (tau != 0) return;
         const uint32_t slot = atomicAdd(&cnt[g], 1u);
         if (slot < (uint32_t)kw) out[(size_t)qpos_ * kw + slot] = sp;
     }
-    __device__ void finish(int, int, bool, uint32_t, float, float, float, const uint32_t*, const uint32_t*) {}
+    __device__ __forceinline__ void finish(int, int, bool, uint32_t, float, float, float, const uint32_t*, const uint32_t*) {}
 };
 
 // hipEvent pairs around kernels on the handle's stream, accumulated per timer.
@@ -566,6 +566,7 @@ struct lvrhip_surface {
         if (n <= 0) return;
         const int64_t N = n;
         const int ns = nscans;
+        const int tok = prof.start(LVRHIP_T_BUILD, st);
         box_init_kernel<<<(unsigned)ceil_div(6 * ns, 256), 256, 0, st>>>(sbox.p, ns);
         LVRHIP_LAUNCH_CHECK();
         const int64_t waves = ceil_div(N, (int64_t)kWave * SB_ITEMS);
@@ -612,6 +613,7 @@ struct lvrhip_surface {
         LVRHIP_LAUNCH_CHECK();
         fill_u32_kernel<<<1, 64, 0, st>>>((uint32_t*)err.p, 1, 0u);
         LVRHIP_LAUNCH_CHECK();
+        prof.stop(tok, st);
     }
 
     // out = the entries i < min(nq, *nq_dev) with flags[i] != 0 (qsel[i] or i), in order;
@@ -674,6 +676,7 @@ struct lvrhip_surface {
             const int tok = prof.start(r == 1 ? LVRHIP_T_NORMALS_FIRST : LVRHIP_T_NORMALS_RETRY, st);
             tile<SinkNormals>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
                               mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+            prof.stop(tok, st);
             PlaneFitParams pf{};
             pf.qsel = qsel;
             pf.nq = nq0;
@@ -699,7 +702,6 @@ struct lvrhip_surface {
             LVRHIP_LAUNCH_CHECK();
             // rejected queries -> the next round's list and count
             if (r < 5) compact_flags(s_flags.p, nq0, nq_dev, qsel, lists[r & 1], d_cnt.p + r + 1);
-            prof.stop(tok, st);
         }
     }
 
@@ -1476,6 +1478,7 @@ int lvrhip_rebuild(lvrhip_t h)
         h->ki_valid = false;
         h->build();
         LVRHIP_CHECK(hipStreamSynchronize(h->st));
+        h->prof.collect();
     });
 }
 

@@ -148,7 +148,7 @@ struct SinkNormals {
     float sx, sy, sz, sxx, sxy, sxz, syy, syz, szz;
     float mnx, mny, mnz, mxx, mxy, mxz;
 
-    __device__ void begin(int g, int s, int64_t qi, uint32_t qpos, float qx, float qy, float qz)
+    __device__ __forceinline__ void begin(int g, int s, int64_t qi, uint32_t qpos, float qx, float qy, float qz)
     {
         qpos_ = qpos;
         qi_ = (uint32_t)qi;
@@ -160,7 +160,7 @@ struct SinkNormals {
         mxx = mxy = mxz = -INFINITY;
     }
 
-    __device__ void accept(int tau, int g, int s, float x, float y, float z, uint64_t key, uint32_t spos,
+    __device__ __forceinline__ void accept(int tau, int g, int s, float x, float y, float z, uint64_t key, uint32_t spos,
                            uint32_t* cnt)
     {
         if (tau == 0) {
@@ -186,7 +186,7 @@ struct SinkNormals {
         }
     }
 
-    __device__ void finish(int g, int s, bool ok, uint32_t qpos, float, float, float, const uint32_t*,
+    __device__ __forceinline__ void finish(int g, int s, bool ok, uint32_t qpos, float, float, float, const uint32_t*,
                            const uint32_t* cnt1)
     {
         sx = group_sum<P>(sx);
@@ -301,9 +301,9 @@ struct SinkSdf {
     uint32_t npts;
     uint32_t qpos_;
 
-    __device__ void begin(int g, int s, int64_t qi, uint32_t qpos, float, float, float) { qpos_ = qpos; }
+    __device__ __forceinline__ void begin(int g, int s, int64_t qi, uint32_t qpos, float, float, float) { qpos_ = qpos; }
 
-    __device__ void accept(int tau, int g, int s, float, float, float, uint64_t key, uint32_t sp, uint32_t* cnt)
+    __device__ __forceinline__ void accept(int tau, int g, int s, float, float, float, uint64_t key, uint32_t sp, uint32_t* cnt)
     {
         if (tau != 0) return;
         const uint32_t slot = atomicAdd(&cnt[g], 1u);
@@ -313,7 +313,7 @@ struct SinkSdf {
         }
     }
 
-    __device__ void finish(int g, int s, bool ok, uint32_t qpos, float qx, float qy, float qz, const uint32_t* cnt0,
+    __device__ __forceinline__ void finish(int g, int s, bool ok, uint32_t qpos, float qx, float qy, float qz, const uint32_t* cnt0,
                            const uint32_t*)
     {
         const int cnt = min((int)cnt0[g], kw);
@@ -361,17 +361,17 @@ struct SinkList {
     int kw;
     uint32_t row_;
 
-    __device__ void begin(int g, int s, int64_t qi, uint32_t qpos, float, float, float)
+    __device__ __forceinline__ void begin(int g, int s, int64_t qi, uint32_t qpos, float, float, float)
     {
         row_ = qorig ? qorig[qpos] : qpos;
     }
-    __device__ void accept(int tau, int g, int s, float, float, float, uint64_t key, uint32_t, uint32_t* cnt)
+    __device__ __forceinline__ void accept(int tau, int g, int s, float, float, float, uint64_t key, uint32_t, uint32_t* cnt)
     {
         if (tau != 0) return;
         const uint32_t slot = atomicAdd(&cnt[g], 1u);
         if (slot < (uint32_t)kw) out[(size_t)row_ * kw + slot] = key;
     }
-    __device__ void finish(int, int, bool, uint32_t, float, float, float, const uint32_t*, const uint32_t*) {}
+    __device__ __forceinline__ void finish(int, int, bool, uint32_t, float, float, float, const uint32_t*, const uint32_t*) {}
 };
 
 }  // namespace lvrhip
