@@ -73,3 +73,38 @@ def test_scan_table_is_checked_without_gpu():
                                           None, C.byref(out)) == 1
     assert lib.lvrhip_graph_launch(None) == 1
     assert lib.lvrhip_autotune(None, None, None, None) == 1
+
+
+def test_cpp_header_compiles_and_links(tmp_path):
+    """include/lvr2_amd/HipSurface.hpp against stand-ins with lvr2's layouts (boost::shared_array
+    -like floatArr, LBPointArray, a 32-byte QueryPoint with a vtable), linked with liblvrhip.so."""
+    import subprocess
+
+    src = tmp_path / "use_hipsurface.cpp"
+    src.write_text(r'''
+#include "lvr2_amd/HipSurface.hpp"
+#include <memory>
+struct Vec { float x, y, z; };
+struct QP { Vec m_position; float m_distance; bool m_invalid; virtual ~QP() {} };
+struct FloatArr { std::shared_ptr<float> p; float* get() const { return p.get(); } };
+template <class T> struct LBPointArray { unsigned int width = 0; unsigned int dim = 1; T* elements = nullptr; };
+static_assert(sizeof(QP) == 32, "QueryPoint stand-in layout");
+int main(int argc, char**) {
+    if (argc < 5) return 0;  // compiled and linked only: running needs a GPU
+    FloatArr pts{std::shared_ptr<float>(new float[30], std::default_delete<float[]>())};
+    lvr2::HipSurface s(pts, 10);
+    LBPointArray<float> lb; lb.width = 10; lb.dim = 3; lb.elements = pts.get();
+    lvr2::HipSurface s2(lb);
+    s.setKn(5); s.setKi(5); s.setKd(3); s.setFlippoint(0, 0, 0);
+    std::string m = "PCA"; s.setMethod(m);
+    s.calculateNormals(); s.interpolateNormals(); s.getNormals(pts);
+    std::vector<QP> q(4); s.distances(q, 0.1f);
+    return 0;
+}
+''')
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.join(ROOT, "lvr2_amd")
+    exe = tmp_path / "use_hipsurface"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", inc, str(src), "-L", libdir, "-llvrhip",
+                        f"-Wl,-rpath,{libdir}", "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
