@@ -164,6 +164,18 @@ int lvrhip_distances(lvrhip_t h, const float* q, size_t m, float voxel, float* o
 int lvrhip_distances_device(lvrhip_t h, const float* d_q, size_t m, float voxel, float* d_proj, float* d_eucl,
                             uint8_t* d_invalid);
 
+/* Differentiation form (no lvr2 counterpart; the reverse mode of AdaptiveKSearchSurface::distance,
+ * tcc:414-457, through calcPlaneIPCAExact, tcc:609-673): for the m positions d_q (device) and
+ * upstream weights d_grad_proj[v] = dL/dproj_v (device), adds dL/dpoints to d_grad_xyz (n*3
+ * floats, input order, device memory, accumulated into).  proj is the value lvrhip_distances
+ * returns for the same positions.  Neighbour sets, adaptive k and flip signs are held fixed
+ * (they are piecewise constant in the positions); the neighbour lists are recomputed, not
+ * stored.  Requires normals from lvrhip_calculate_normals with the PCA method and without
+ * interpolation (ki = 0), on a single-scan handle.  Runs on the handle's stream and
+ * synchronises it. */
+int lvrhip_distances_backward_device(lvrhip_t h, const float* d_q, size_t m, const float* d_grad_proj,
+                                     float* d_grad_xyz);
+
 /* Statistics of the last calculate_normals call: n_round[r] = number of points whose
  * neighbourhood search ran with k = kn * 2^(r+1), r = 0..4.  No lvr2 counterpart. */
 int lvrhip_normals_stats(lvrhip_t h, int64_t n_round[5]);

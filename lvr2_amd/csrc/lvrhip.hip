@@ -6,6 +6,7 @@
 #include "lbvh.hpp"
 #include "knn_tile.hpp"
 #include "sinks.hpp"
+#include "adjoint.hpp"
 
 #include "lvrhip.h"
 
@@ -386,6 +387,10 @@ struct lvrhip_surface {
 
     DBuf<float> nx, ny, nz, tx, ty, tz;
     bool has_normals = false;
+    bool interpolated = false;  // nx/ny/nz hold interpolated normals
+    // reverse pass
+    DBuf<float> g_p, g_n, adj;
+    DBuf<uint32_t> a_flags, a_sel, a_list;
     DBuf<uint32_t> ki_list;
     int kiw = 0;
     bool ki_valid = false;
@@ -709,6 +714,7 @@ struct lvrhip_surface {
     {
         drop_graph();
         has_normals = false;
+        interpolated = false;
         ki_valid = false;
         for (auto& c : n_round) c = 0;
         if (n == 0) {
@@ -784,6 +790,7 @@ struct lvrhip_surface {
         tz.reserve(npad);
         enqueue_interpolate(tx.p, ty.p, tz.p);
         if (prof.on) prof.collect();
+        interpolated = true;
         std::swap(nx.p, tx.p);
         std::swap(nx.cap, tx.cap);
         std::swap(ny.p, ty.p);
@@ -1028,6 +1035,97 @@ struct lvrhip_surface {
         prof.collect();
     }
 
+    // Reverse pass of distances: d_grad (n x 3, input order) += d(sum_v w_v proj_v)/d(points)
+    // with neighbour sets, adaptive k and flip signs held fixed.
+    void distances_backward_device(const float* d_q, int64_t m, const float* d_w, float* d_grad)
+    {
+        if (!has_normals) throw Error(LVRHIP_ESTATE, "distances_backward called before calculate_normals");
+        require_one_scan("distances_backward");
+        if (method != LVRHIP_METHOD_PCA) throw Error(LVRHIP_EINVAL, "the adjoint is provided for the PCA method only");
+        if (interpolated) throw Error(LVRHIP_ESTATE, "the adjoint needs uninterpolated normals (ki = 0)");
+        if (m == 0 || n == 0) return;
+        drop_graph();
+        const int64_t R = npad;
+        const int t0 = clamp_k(kd);
+        s_k1.reserve(std::max<size_t>((size_t)m * t0, (size_t)m));
+        s_spos.reserve((size_t)m * t0);
+        s_ord.reserve((size_t)m * t0);
+        g_p.reserve(3 * (size_t)R);
+        g_n.reserve(3 * (size_t)R);
+        adj.reserve((size_t)R * ADJ_STRIDE);
+        a_flags.reserve(R);
+        a_sel.reserve(R);
+        a_list.reserve(R);
+        d_cnt.reserve(8);
+        zero_f32_kernel<<<(unsigned)ceil_div(3 * R, 256), 256, 0, st>>>(g_p.p, 3 * R);
+        LVRHIP_LAUNCH_CHECK();
+        zero_f32_kernel<<<(unsigned)ceil_div(3 * R, 256), 256, 0, st>>>(g_n.p, 3 * R);
+        LVRHIP_LAUNCH_CHECK();
+        sort_queries(d_q, m);
+        // 1. SDF reverse: dL/dp (through the means) and dL/dn of every vertex's kd neighbours
+        {
+            TileParams prm = params(t0, 0, 2);
+            prm.stats = kstats_ptr(2);
+            QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
+            auto mk = [&](auto tag) {
+                constexpr int P = decltype(tag)::value;
+                SinkSdfAdjoint<P> s{};
+                s.px = px.p;
+                s.py = py.p;
+                s.pz = pz.p;
+                s.nx = nx.p;
+                s.ny = ny.p;
+                s.nz = nz.p;
+                s.qorig = s_v0.p;
+                s.w = d_w;
+                s.keys = s_k1.p;
+                s.spos = s_spos.p;
+                s.ord = s_ord.p;
+                s.kw = t0;
+                s.gpx = g_p.p;
+                s.gpy = g_p.p + R;
+                s.gpz = g_p.p + 2 * R;
+                s.gnx = g_n.p;
+                s.gny = g_n.p + R;
+                s.gnz = g_n.p + 2 * R;
+                s.npts = (uint32_t)R;
+                return s;
+            };
+            tile<SinkSdfAdjoint>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
+                                 mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+        }
+        // 2. dL/dC per point with a normal gradient
+        pca_adjoint_prep_kernel<<<(unsigned)ceil_div(R, 256), 256, 0, st>>>(
+            R, px.p, py.p, pz.p, nx.p, ny.p, nz.p, mom.p, kround.p, kn, nscans == 1 ? n : (int64_t)1 << 40, g_n.p,
+            g_n.p + R, g_n.p + 2 * R, adj.p, a_flags.p);
+        LVRHIP_LAUNCH_CHECK();
+        // 3. PCA reverse, one launch per adaptive round (the forward's k-sets)
+        for (int r = 1; r <= 5; r++) {
+            select_round_kernel<<<(unsigned)ceil_div(R, 256), 256, 0, st>>>(a_flags.p, R, (uint32_t)(1 + r), a_sel.p);
+            LVRHIP_LAUNCH_CHECK();
+            compact_flags(a_sel.p, R, nullptr, nullptr, a_list.p, d_cnt.p + 6);
+            TileParams prm = params(clamp_k((int64_t)kn << r), 0, 3);
+            prm.stats = kstats_ptr(3);
+            QueryView qv{px.p, py.p, pz.p, a_list.p, (uint32_t)R, d_cnt.p + 6};
+            auto mk = [&](auto tag) {
+                constexpr int P = decltype(tag)::value;
+                SinkPcaAdjoint<P> s{};
+                s.adj = adj.p;
+                s.gpx = g_p.p;
+                s.gpy = g_p.p + R;
+                s.gpz = g_p.p + 2 * R;
+                return s;
+            };
+            tile<SinkPcaAdjoint>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
+                                 mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+        }
+        accumulate_grad_kernel<<<(unsigned)ceil_div(R, 256), 256, 0, st>>>(g_p.p, g_p.p + R, g_p.p + 2 * R, pidx.p, R,
+                                                                         d_grad);
+        LVRHIP_LAUNCH_CHECK();
+        check_err();
+        prof.collect();
+    }
+
     void knn_device(const float* d_q, int64_t m, int k, int32_t* d_idx, float* d_d2)
     {
         require_one_scan("kSearch");
@@ -1255,6 +1353,7 @@ int lvrhip_set_normals(lvrhip_t h, const float* normals)
         h->drop_graph();
         h->ki_valid = false;
         h->has_normals = false;
+        h->interpolated = false;
         if (N > 0) {
             const int64_t R = h->npad;
             h->nx.reserve(R);
@@ -1358,6 +1457,19 @@ int lvrhip_distances(lvrhip_t h, const float* q, size_t m, float voxel, float* o
         if (out_invalid)
             LVRHIP_CHECK(hipMemcpyAsync(out_invalid, di.p, m * sizeof(uint8_t), hipMemcpyDeviceToHost, h->st));
         LVRHIP_CHECK(hipStreamSynchronize(h->st));
+    });
+}
+
+int lvrhip_distances_backward_device(lvrhip_t h, const float* d_q, size_t m, const float* d_grad_proj,
+                                     float* d_grad_xyz)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        REQUIRE(m == 0 || (d_q && d_grad_proj), "NULL query or gradient pointer");
+        REQUIRE(h->n == 0 || d_grad_xyz, "NULL output gradient pointer");
+        REQUIRE(m < (size_t)0xFFFFFFFFu, "too many queries");
+        DeviceGuard dg(h->device);
+        h->distances_backward_device(d_q, (int64_t)m, d_grad_proj, d_grad_xyz);
     });
 }
 

@@ -121,3 +121,21 @@ def scan_project(nscans: int = 64, lo: int = 50_000, hi: int = 2_000_000, seed: 
         flips.append(pos)
     return ScanProject(f"scan_project_{nscans}", np.concatenate(pts), np.asarray(sizes, np.int64),
                        np.asarray(flips, np.float32), kn, ki, kd)
+
+
+def band_queries(points: np.ndarray, m: int, width: float, seed: int = 6) -> np.ndarray:
+    """m query positions within `width` of the cloud: a random point of the cloud plus a
+    uniform random offset in the ball of radius `width`."""
+    rng = np.random.default_rng(seed)
+    base = points[rng.integers(0, len(points), m)].astype(np.float64)
+    d = rng.normal(size=(m, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = width * rng.uniform(0, 1, m) ** (1.0 / 3.0)
+    return (base + d * r[:, None]).astype(np.float32)
+
+
+def sdf_denoise(n: int = 1_000_000, m: int = 500_000, seed: int = 5):
+    """The differentiation workload: the noisy sphere + torus (sigma 1 % of R), kn 30, kd 10,
+    voxel 0.05 R, m query positions in a 2-voxel band."""
+    w = sphere_torus_1m(n, seed)
+    return w, band_queries(w.points, m, 2 * w.voxel, seed + 1)

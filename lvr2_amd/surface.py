@@ -192,6 +192,23 @@ class HipSurface:
         check(lib().lvrhip_get_tuning(self._h, C.byref(p), C.byref(t), C.byref(m)))
         return {"lanes_per_query": p.value, "tiles_per_wave": t.value, "r2_margin": m.value}
 
+    def distancesBackward(self, queries, grad_proj, grad_points=None):
+        """Reverse mode of ``distances``' proj w.r.t. the points (device tensors only):
+        returns grad_points (N, 3) += d(sum grad_proj * proj)/d(points)."""
+        import torch
+
+        if not (_is_device_tensor(queries) and _is_device_tensor(grad_proj)):
+            raise ValueError("queries and grad_proj must be device tensors")
+        q = queries.detach().to(torch.float32).contiguous()
+        w = grad_proj.detach().to(torch.float32).contiguous()
+        if w.shape != (q.shape[0],):
+            raise ValueError("one weight per query")
+        if grad_points is None:
+            grad_points = torch.zeros((self._n, 3), dtype=torch.float32, device=q.device)
+        check(lib().lvrhip_distances_backward_device(self._h, C.c_void_p(q.data_ptr()), int(q.shape[0]),
+                                                     C.c_void_p(w.data_ptr()), C.c_void_p(grad_points.data_ptr())))
+        return grad_points
+
     # -- rebuild and graphs -----------------------------------------------------------------------
     def rebuild(self) -> None:
         """Rebuild the LBVH from the point buffer (after the points were changed in place)."""
