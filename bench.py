@@ -111,6 +111,141 @@ def cpu_baseline(w, gpu_normals, verts, sample_pts: int, sample_q: int):
     }
 
 
+def bench_batched(args):
+    """The batched form on the scan project: 64 scans of one room from random scanner positions,
+    sizes log-uniform in [50,000, 2,000,000], per-scan flip points, kn 50 ki 20.  Compares one
+    handle per scan (the reference's per-scan loop, ScanProjectUtils.cpp:361-410) with one
+    batched handle, eager and replayed from a hipGraph (LBVH build included in every variant)."""
+    import torch
+
+    from lvr2_amd import HipSurface, scenes
+
+    log("[bench] generating the scan project")
+    sp = scenes.scan_project(args.scans)
+    n = len(sp.points)
+    dev = torch.device("cuda:0")
+    pts = torch.from_numpy(sp.points).to(dev)
+    out = torch.empty_like(pts)
+    offs = np.concatenate([[0], np.cumsum(sp.sizes)])
+    log(f"[bench] {len(sp.sizes)} scans, {n} points")
+
+    def per_scan():
+        for s_ in range(len(sp.sizes)):
+            h = HipSurface(pts[offs[s_]:offs[s_ + 1]])
+            h.setKn(sp.kn)
+            h.setKi(sp.ki)
+            h.setFlippoint(*sp.flips[s_])
+            h.calculateNormals()
+            h.interpolateNormals()
+            h.getNormals(out[offs[s_]:offs[s_ + 1]])
+            h.freeGPU()
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    t_loop = timed(per_scan, 1)
+    ref = out.clone()
+    log(f"[bench] per-scan handles: {t_loop * 1e3:.1f} ms")
+    b = HipSurface(pts, scan_sizes=sp.sizes)
+    b.setKn(sp.kn)
+    b.setKi(sp.ki)
+    b.setScanFlippoints(sp.flips)
+
+    def eager():
+        b.rebuild()
+        b.calculateNormals()
+        b.interpolateNormals()
+        b.getNormals(out)
+
+    t_eager = timed(eager, args.steps)
+    same = bool(torch.equal(out, ref))
+    log(f"[bench] batched eager: {t_eager * 1e3:.1f} ms (bit-equal to per-scan: {same})")
+    tune = b.autotune()
+    log(f"[bench] autotuned launch shape: {tune}")
+    b.captureGraph(out, rebuild=True)
+
+    def replay():
+        b.launchGraph()
+        b.synchronize()
+
+    t_graph = timed(replay, args.steps)
+    log(f"[bench] batched graph replay: {t_graph * 1e3:.1f} ms")
+    print(json.dumps({
+        "metric": "batched normals Mpts/s (LBVH build + adaptive kNN/PCA + interpolation, all scans)",
+        "form": "batched", "unit": "Mpts/s", "n_gpus": 1, "points": n, "scans": len(sp.sizes),
+        "per_scan_handles": round(n / t_loop / 1e6, 2), "batched_eager": round(n / t_eager / 1e6, 2),
+        "batched_graph": round(n / t_graph / 1e6, 2), "graph_vs_per_scan": round(t_loop / t_graph, 2),
+        "ms": {"per_scan": round(t_loop * 1e3, 1), "eager": round(t_eager * 1e3, 1), "graph": round(t_graph * 1e3, 1)},
+        "autotuned": tune, "eager_bit_equal_to_per_scan": same, "data": "synthetic (seeded ray-cast room scans)",
+    }), flush=True)
+
+
+def bench_adjoint(args):
+    """The differentiation form: 1,000,000-point noisy sphere + torus, kn 30 kd 10, loss = sum of
+    squared Hoppe SDF at 500,000 band queries; forward alone vs forward + adjoint, then 20
+    gradient-descent denoising steps."""
+    import torch
+
+    from lvr2_amd import scenes
+    from lvr2_amd.autograd import hoppe_sdf
+
+    w, q = scenes.sdf_denoise()
+    dev = torch.device("cuda:0")
+    qt = torch.from_numpy(q).to(dev)
+    x0 = torch.from_numpy(w.points).to(dev)
+
+    def fwd(x):
+        return hoppe_sdf(x, qt, kn=w.kn, kd=w.kd, flip=w.flip, voxel=w.voxel)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    t_f = timed(lambda: fwd(x0), args.steps)
+
+    def fb():
+        x = x0.detach().requires_grad_(True)
+        (fwd(x) ** 2).sum().backward()
+        return x.grad
+
+    t_fb = timed(fb, args.steps)
+    losses = []
+    x = x0.clone()
+    l0 = float((fwd(x) ** 2).sum())
+    lr = 0.2 * w.voxel ** 2 / max(l0, 1e-30) * len(q)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(20):
+        x = x.detach().requires_grad_(True)
+        loss = (fwd(x) ** 2).sum()
+        loss.backward()
+        losses.append(float(loss.detach()))
+        with torch.no_grad():
+            x = x - lr * x.grad
+    torch.cuda.synchronize()
+    t_steps = (time.perf_counter() - t) / 20
+    log(f"[bench] adjoint: forward {t_f * 1e3:.1f} ms, forward+backward {t_fb * 1e3:.1f} ms, step {t_steps * 1e3:.1f} ms")
+    print(json.dumps({
+        "metric": "SDF adjoint: forward + backward ms (1M points, 500k queries) vs forward alone",
+        "form": "adjoint", "unit": "ms", "n_gpus": 1, "points": len(w.points), "queries": len(q),
+        "forward_ms": round(t_f * 1e3, 2), "forward_backward_ms": round(t_fb * 1e3, 2),
+        "backward_over_forward": round((t_fb - t_f) / t_f, 2), "denoise_step_ms": round(t_steps * 1e3, 2),
+        "loss_first": losses[0], "loss_last": losses[-1], "steps": 20,
+        "data": "synthetic (seeded noisy sphere + torus)",
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +259,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel (y = x * 1) over MIB MiB three times: the known-size "
                          "reference for HBM counter scaling")
+    ap.add_argument("--form", choices=("plain", "batched", "adjoint"), default="plain",
+                    help="plain: the room-scan headline line; batched / adjoint: the two other forms")
+    ap.add_argument("--scans", type=int, default=64)
     args = ap.parse_args()
+    if args.form != "plain" and args.gpus == 1:
+        (bench_batched if args.form == "batched" else bench_adjoint)(args)
+        return
     if args.copy_calibration:
         import torch
 

@@ -180,6 +180,12 @@ int lvrhip_distances_backward_device(lvrhip_t h, const float* d_q, size_t m, con
  * neighbourhood search ran with k = kn * 2^(r+1), r = 0..4.  No lvr2 counterpart. */
 int lvrhip_normals_stats(lvrhip_t h, int64_t n_round[5]);
 
+/* The neighbourhood size each point's normal was fitted on by the last calculate_normals:
+ * the k of its adaptive round (kn * 2^r, r = 1..5, at most the scan's size), the k at which
+ * calculateSurfaceNormals' loop stopped (AdaptiveKSearchSurface.tcc:187-216).  n int32, input
+ * order.  No lvr2 counterpart (the loop's k is local there). */
+int lvrhip_get_adaptive_k(lvrhip_t h, int32_t* out_k);
+
 /* Number of points of the handle. */
 int lvrhip_size(lvrhip_t h, size_t* n);
 
@@ -216,8 +222,18 @@ int lvrhip_get_timings(lvrhip_t h, double* ms, int64_t* launches, int reset);
 /* Autotuner (no lvr2 counterpart): times the adaptive normals pass on a sample of the
  * cloud (every 16th run of 4096 points in Morton order) for lanes per query in {4, 8},
  * tiles per wavefront in {4, 8, 16} and radius margins in {1.15, 1.3, 1.6}, keeps the
- * fastest launch shape and returns it (any output may be NULL).  Invalidates the normals. */
+ * fastest launch shape and returns it (any output may be NULL), then keeps the faster of the
+ * two retry modes (lvrhip_set_retry_mode) at that shape.  Invalidates the normals. */
 int lvrhip_autotune(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float* r2_margin);
+
+/* How calculate_normals runs the adaptive rounds 2..5 for the points round 1 rejected (no lvr2
+ * counterpart; the results are the same either way): merged != 0 searches k = 32 kn once with
+ * the four round sizes as targets, takes the first round whose neighbourhood passes
+ * boundingBoxOK and searches the points that stop before round 5 again at their round's k;
+ * merged == 0 (the default) searches round by round, each round only for the points the
+ * previous one rejected.  lvrhip_autotune also chooses this. */
+int lvrhip_set_retry_mode(lvrhip_t h, int merged);
+int lvrhip_get_retry_mode(lvrhip_t h, int* merged);
 
 /* The current launch shape (any output may be NULL). */
 int lvrhip_get_tuning(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float* r2_margin);

@@ -47,6 +47,9 @@ _SIGS = {
     "lvrhip_synchronize": (C.c_int, [C.c_void_p]),
     "lvrhip_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "lvrhip_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "lvrhip_set_retry_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "lvrhip_get_adaptive_k": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lvrhip_get_retry_mode": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "lvrhip_distances_backward_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
 }
 

@@ -28,6 +28,7 @@ namespace lvrhip {
 // neighbours (atomics into sorted-position arrays).
 template <int P>
 struct SinkSdfAdjoint {
+    static constexpr int kTargets = 2;
     const float* px;
     const float* py;
     const float* pz;
@@ -189,6 +190,7 @@ __global__ void select_round_kernel(const uint32_t* __restrict__ flags, int64_t 
 // C_j (the k-set, and the point itself once more as the seed) receives (2/n) dL/dC (x_s - mu).
 template <int P>
 struct SinkPcaAdjoint {
+    static constexpr int kTargets = 2;
     const float* adj;
     float* gpx;
     float* gpy;

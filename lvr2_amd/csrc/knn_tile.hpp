@@ -20,8 +20,8 @@
 //      bin to the sink, buffers the bin, and ranks the buffer exactly.
 // Every point within R of a query is a candidate (box distances are monotone lower bounds of
 // the fp32 d^2), so a query with >= t in-range candidates gets the exact t smallest keys.
-// Two targets are selected at once (e.g. k for the plane fit and ki for the interpolation
-// list).  If a tile's leaf list outgrows LDS, each pass re-walks the tree and streams the
+// Two or four targets are selected at once (e.g. k for the plane fit and ki for the
+// interpolation list, or the four neighbourhood sizes of the later adaptive rounds).  If a tile's leaf list outgrows LDS, each pass re-walks the tree and streams the
 // leaves chunk by chunk instead.
 #pragma once
 
@@ -62,7 +62,9 @@ struct QueryView {
 };
 
 struct TileParams {
-    int t0, t1;  // targets (t1 <= t0; t1 = 0 for none), already clamped to n
+    int t0, t1, t2, t3;  // targets, already clamped to n: t0 the largest (it decides whether a
+                         // radius suffices), the others <= t0 (0 for none); a sink uses
+                         // Sink::kTargets of them
     int tiles_per_wave;
     float r2_init;
     const float* area_pp;       // optional device value, area per point: first radius^2 =
@@ -118,7 +120,7 @@ struct WaveStats {
     }
 };
 
-template <int Q, int BUF>
+template <int Q, int BUF, int NT>
 struct TileSmem {
     float4 cand[CAP];  // x, y, z, original index (as bits)
     float4 lbox[CHUNK_LEAVES][2];  // boxes of the chunk's leaves (lo, hi)
@@ -126,10 +128,10 @@ struct TileSmem {
     int stack[STACK_CAP];
     int tleaf[TL_CAP];                // the tile's leaves (or the current chunk's, when streaming)
     uint8_t glist[Q][CHUNK_LEAVES];   // per query: chunk leaf slots within its bound
-    uint64_t bkey[2][Q][BUF];
-    uint32_t bspos[2][Q][BUF];
-    uint32_t bcnt[2][Q];
-    uint32_t scnt[2][Q];  // per-query sink slot counters
+    uint64_t bkey[NT][Q][BUF];
+    uint32_t bspos[NT][Q][BUF];
+    uint32_t bcnt[NT][Q];
+    uint32_t scnt[NT][Q];  // per-query sink slot counters
 };
 
 // ---- traversal ---------------------------------------------------------------------------
@@ -435,7 +437,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
 {
     constexpr int Q = kWave / P;
     constexpr int BUF = 12;
-    __shared__ TileSmem<Q, BUF> sm;
+    constexpr int NT = Sink::kTargets;
+    static_assert(NT == 2 || NT == 4, "two or four targets");
+    __shared__ TileSmem<Q, BUF, NT> sm;
     const int lane = lane_id();
     const int g = lane / P, s = lane % P;
     float r2 = fmaxf(prm.r2_init, 1e-30f);
@@ -465,8 +469,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
         ws.c[KS_TILES] += 1;
         sink.begin(g, s, qi, qpos, qx, qy, qz);
         if (s == 0) {
-            sm.scnt[0][g] = 0u;
-            sm.scnt[1][g] = 0u;
+#pragma unroll
+            for (int tau = 0; tau < NT; tau++) sm.scnt[tau][g] = 0u;
         }
         float kth_max = 0.0f;
 
@@ -474,9 +478,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             const uint64_t pending = ballot(active);
             if (pending == 0) break;
             // One scan per attempt (the scan of the lowest pending lane): neighbourhoods never
-            // cross scans, each scan's points form one subtree of the hierarchy.
-            const int cur = bv.scan_of ? uniform(shfl(scan, __ffsll((long long)pending) - 1)) : 0;
-            const bool act = active && scan == cur;
+            // cross scans, each scan's points form one subtree of the hierarchy.  And only the
+            // pending queries within R (per axis) of that lane's query: a tile of scattered
+            // queries (a sparse query list) would otherwise span a box far larger than R.
+            const int lead = __ffsll((long long)pending) - 1;
+            const int cur = bv.scan_of ? uniform(shfl(scan, lead)) : 0;
+            const float lx = uniformf(shfl(qx, lead)), ly = uniformf(shfl(qy, lead)), lz = uniformf(shfl(qz, lead));
+            const float reach = sqrtf(r2);
+            const bool near = fabsf(qx - lx) <= reach && fabsf(qy - ly) <= reach && fabsf(qz - lz) <= reach;
+            const bool act = active && scan == cur && near;
             const int root = bv.roots ? uniform(bv.roots[cur]) : (bv.nleaf > 1 ? 0 : -1);
             // tile box of the pending queries of that scan
             TileBox tb;
@@ -530,22 +540,22 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
 
             // --- targets -----------------------------------------------------------------------
             const auto t_sel = ws.tick();
-            Target tg[2];
+            Target tg[NT];
             bool ok = false;
             {
                 int b, below, cnt;
                 const int total = find_bin<P, Q>(sm.hist, g, s, prm.t0, b, below, cnt);
                 ok = act && total >= prm.t0;
 #pragma unroll
-                for (int tau = 0; tau < 2; tau++) {
-                    const int t = tau == 0 ? prm.t0 : prm.t1;
+                for (int tau = 0; tau < NT; tau++) {
+                    const int t = tau == 0 ? prm.t0 : tau == 1 ? prm.t1 : tau == 2 ? prm.t2 : prm.t3;
                     tg[tau].t = t;
                     tg[tau].refine = false;
                     tg[tau].below = 0;
                     tg[tau].cnt = 0;
                     tg[tau].klo = ~0ull;
                     tg[tau].khi = ~0ull;
-                    if (tau == 1) {
+                    if (tau >= 1) {
                         if (t > 0) find_bin<P, Q>(sm.hist, g, s, t, b, below, cnt);
                     }
                     if (ok && t > 0) {
@@ -566,7 +576,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
 
             // --- refinement passes (radix over the 64-bit key), one target at a time -----------
 #pragma unroll
-            for (int tau = 0; tau < 2; tau++) {
+            for (int tau = 0; tau < NT; tau++) {
                 for (int guard = 0; guard < 16; guard++) {
                     const bool need = ok && tg[tau].refine;
                     if (ballot(need) == 0) break;
@@ -605,21 +615,22 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                     }
                 }
             }
+            float rc = -1.0f;  // collection bound: the largest boundary bin edge
 #pragma unroll
-            for (int tau = 0; tau < 2; tau++) {
+            for (int tau = 0; tau < NT; tau++) {
                 tg[tau].dlo = bitsf((uint32_t)(tg[tau].klo >> 32));
                 tg[tau].dhi = bitsf((uint32_t)(tg[tau].khi >> 32));
+                if (ok && tg[tau].t > 0) rc = fmaxf(rc, tg[tau].dhi);
             }
 
             ws.tock(KS_CY_SELECT, t_sel);
 
             // --- collection pass, bounded by the last bin's upper edge ---------------------------------
             if (s == 0) {
-                sm.bcnt[0][g] = 0;
-                sm.bcnt[1][g] = 0;
+#pragma unroll
+                for (int tau = 0; tau < NT; tau++) sm.bcnt[tau][g] = 0;
             }
             __syncthreads();
-            const float rc = ok ? fmaxf(tg[0].dhi, prm.t1 > 0 ? tg[1].dhi : 0.0f) : -1.0f;
             auto collect = [&](int nl, int base) {
                 auto t0 = ws.tick();
                 const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rc);
@@ -629,7 +640,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                     const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
                     if (!(d2 <= R2)) return;
 #pragma unroll
-                    for (int tau = 0; tau < 2; tau++) {
+                    for (int tau = 0; tau < NT; tau++) {
                         if (tg[tau].t <= 0) continue;
                         if (d2 < tg[tau].dlo) {
                             sink.accept(tau, g, s, p.x, p.y, p.z, make_key(d2, __float_as_uint(p.w)),
@@ -661,7 +672,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             float kth = 0.0f;
             if (ok) {
 #pragma unroll
-                for (int tau = 0; tau < 2; tau++) {
+                for (int tau = 0; tau < NT; tau++) {
                     if (tg[tau].t <= 0) continue;
                     const int nb = min((int)sm.bcnt[tau][g], BUF);
                     const int need = tg[tau].t - tg[tau].below;

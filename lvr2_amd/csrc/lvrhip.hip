@@ -233,6 +233,18 @@ __global__ void scatter_normals_kernel(const float* __restrict__ nx, const float
     out[3 * (int64_t)j + 2] = nz[i];
 }
 
+// Adaptive neighbourhood size per point, input order: min(kn << round, kmax).
+__global__ void adaptive_k_kernel(const uint8_t* __restrict__ kround, const uint32_t* __restrict__ pidx, int64_t n,
+                                  int kn, int64_t kmax, int32_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = pidx[i];
+    if (j == ~0u) return;
+    const int64_t k = (int64_t)kn << kround[i];
+    out[j] = (int32_t)(k < kmax ? k : kmax);
+}
+
 // Autotuning sample: every `every`-th run of `run` consecutive real positions (sort order).
 __global__ void sample_runs_kernel(const uint32_t* __restrict__ qreal, int64_t m, int run, int every,
                                    uint32_t* __restrict__ out)
@@ -268,6 +280,7 @@ __global__ __launch_bounds__(kWave) void sort_rows_kernel(const uint64_t* __rest
 // unordered, like the one written by SinkNormals.
 template <int P>
 struct SinkKiOnly {
+    static constexpr int kTargets = 2;
     uint32_t* out;
     int kw;
     uint32_t qpos_;
@@ -370,6 +383,7 @@ struct lvrhip_surface {
     int lanes_per_query = 8;
     int tiles_per_wave = 8;
     float r2_margin = 1.3f;
+    bool merge_retry = false;  // rounds 2..5 in one four-target search
 
     DBuf<float> px, py, pz;
     DBuf<uint32_t> pidx, rank, qreal;
@@ -650,6 +664,8 @@ struct lvrhip_surface {
         s_flags.reserve(n);
         s_list0.reserve(n);
         s_list1.reserve(n);
+        a_sel.reserve(n);
+        a_list.reserve(n);
         d_cnt.reserve(8);
     }
 
@@ -662,7 +678,8 @@ struct lvrhip_surface {
         uint32_t* lists[2] = {s_list0.p, s_list1.p};
         nq_first = nq0;
         if (nq0 == 0) return;
-        for (int r = 1; r <= 5; r++) {
+        const int last = merge_retry ? 1 : 5;  // rounds searched one at a time
+        for (int r = 1; r <= last; r++) {
             const int t0 = clamp_k((int64_t)kn << r);
             TileParams prm = params(t0, r == 1 ? kiw : 0, 0);
             prm.stats = kstats_ptr(r == 1 ? 0 : 1);
@@ -708,6 +725,93 @@ struct lvrhip_surface {
             // rejected queries -> the next round's list and count
             if (r < 5) compact_flags(s_flags.p, nq0, nq_dev, qsel, lists[r & 1], d_cnt.p + r + 1);
         }
+        if (merge_retry) enqueue_merged_rounds(lists[1], d_cnt.p + 2, nq0);
+    }
+
+    // Rounds 2..5 for the queries round 1 rejected, in one search with four targets
+    // (SinkLevels); the plane fit of the round-5 points; the points that stop at round 2..4
+    // searched again at their round's k (SinkNormals) and fitted.
+    void enqueue_merged_rounds(const uint32_t* qsel, const uint32_t* nq_dev, int64_t nq0)
+    {
+        TileParams prm = params(clamp_k((int64_t)kn << 5), clamp_k((int64_t)kn << 4), 1);
+        prm.t2 = clamp_k((int64_t)kn << 3);
+        prm.t3 = clamp_k((int64_t)kn << 2);
+        prm.stats = kstats_ptr(1);
+        uint32_t* redo = s_flags.p;
+        {
+            QueryView qv{px.p, py.p, pz.p, qsel, (uint32_t)nq0, nq_dev};
+            auto mk = [&](auto tag) {
+                constexpr int P = decltype(tag)::value;
+                SinkLevels<P> s{};
+                s.mom = mom.p;
+                s.kround = kround.p;
+                s.redo = redo;
+                return s;
+            };
+            const int tok = prof.start(LVRHIP_T_NORMALS_RETRY, st);
+            tile<SinkLevels>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
+                             mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+            prof.stop(tok, st);
+        }
+        plane_fit(qsel, nq0, nq_dev, redo, prm.t0, 5);
+        fill_u32_kernel<<<1, 64, 0, st>>>(d_cnt.p + 3, 3, 0u);
+        LVRHIP_LAUNCH_CHECK();
+        round_count_kernel<<<(unsigned)ceil_div(nq0, 256), 256, 0, st>>>(qsel, nq_dev, nq0, kround.p, d_cnt.p);
+        LVRHIP_LAUNCH_CHECK();
+        // points whose search stops at round 2, 3 or 4
+        for (int r = 2; r <= 4; r++) {
+            select_round_kernel<<<(unsigned)ceil_div(nq0, 256), 256, 0, st>>>(redo, nq0, (uint32_t)r, a_sel.p);
+            LVRHIP_LAUNCH_CHECK();
+            compact_flags(a_sel.p, nq0, nq_dev, qsel, a_list.p, d_cnt.p + 6);
+            const int t0 = clamp_k((int64_t)kn << r);
+            TileParams pr = params(t0, 0, 1);
+            pr.stats = kstats_ptr(1);
+            QueryView qv{px.p, py.p, pz.p, a_list.p, (uint32_t)nq0, d_cnt.p + 6};
+            auto mk = [&](auto tag) {
+                constexpr int P = decltype(tag)::value;
+                SinkNormals<P> s{};
+                s.mom = mom.p;
+                s.retry = nullptr;
+                s.ki_list = ki_list.p;
+                s.kiw = 0;
+                return s;
+            };
+            const int tok = prof.start(LVRHIP_T_NORMALS_RETRY, st);
+            tile<SinkNormals>(qv, pr, lanes_per_query, mk(std::integral_constant<int, 2>()),
+                              mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+            prof.stop(tok, st);
+            plane_fit(a_list.p, nq0, d_cnt.p + 6, nullptr, t0, r);
+        }
+    }
+
+    void plane_fit(const uint32_t* qsel, int64_t nq, const uint32_t* nq_dev, const uint32_t* skip, int kcount,
+                   int round)
+    {
+        PlaneFitParams pf{};
+        pf.qsel = qsel;
+        pf.nq = nq;
+        pf.nq_dev = nq_dev;
+        pf.retry = skip;
+        pf.mom = mom.p;
+        pf.px = px.p;
+        pf.py = py.p;
+        pf.pz = pz.p;
+        pf.method = method;
+        pf.kcount = kcount;
+        pf.kn = kn;
+        pf.kmax = nscans == 1 ? n : ((int64_t)1 << 40);
+        pf.fx = flip[0];
+        pf.fy = flip[1];
+        pf.fz = flip[2];
+        pf.flips = (scan_flips && nscans > 1) ? d_flips.p : nullptr;
+        pf.scan_of = nscans > 1 ? scan_of.p : nullptr;
+        pf.nx = nx.p;
+        pf.ny = ny.p;
+        pf.nz = nz.p;
+        pf.kround = kround.p;
+        pf.round = round;
+        plane_fit_kernel<<<(unsigned)ceil_div(nq, 256), 256, 0, st>>>(pf);
+        LVRHIP_LAUNCH_CHECK();
     }
 
     void calculate_normals()
@@ -952,11 +1056,31 @@ struct lvrhip_surface {
             (void)hipEventDestroy(b);
             throw;
         }
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
         lanes_per_query = bp;
         tiles_per_wave = bt;
         r2_margin = bm;
+        // the later rounds merged into one search, or searched one at a time
+        try {
+            const bool cur = merge_retry;
+            merge_retry = !cur;
+            float t = INFINITY;
+            for (int rep = 0; rep < 2; rep++) {
+                LVRHIP_CHECK(hipEventRecord(a, st));
+                enqueue_normals(s_sample.p, m);
+                LVRHIP_CHECK(hipEventRecord(b, st));
+                read_err_and_counts();
+                float ms = 0.0f;
+                LVRHIP_CHECK(hipEventElapsedTime(&ms, a, b));
+                t = std::min(t, ms);
+            }
+            if (!(t < best)) merge_retry = cur;
+        } catch (...) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            throw;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
         has_normals = false;
         ki_valid = false;
         for (auto& c : n_round) c = 0;
@@ -1481,6 +1605,24 @@ int lvrhip_normals_stats(lvrhip_t h, int64_t n_round[5])
     });
 }
 
+int lvrhip_get_adaptive_k(lvrhip_t h, int32_t* out_k)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        REQUIRE(h->has_normals && h->kround.p, "no adaptive rounds: call lvrhip_calculate_normals first");
+        if (h->n == 0) return;
+        REQUIRE(out_k, "output pointer is NULL");
+        DeviceGuard dg(h->device);
+        DBuf<int32_t> d;
+        d.reserve((size_t)h->n);
+        adaptive_k_kernel<<<(unsigned)ceil_div(h->npad, 256), 256, 0, h->st>>>(
+            h->kround.p, h->pidx.p, h->npad, h->kn, h->nscans == 1 ? h->n : ((int64_t)1 << 40), d.p);
+        LVRHIP_LAUNCH_CHECK();
+        LVRHIP_CHECK(hipMemcpyAsync(out_k, d.p, (size_t)h->n * sizeof(int32_t), hipMemcpyDeviceToHost, h->st));
+        LVRHIP_CHECK(hipStreamSynchronize(h->st));
+    });
+}
+
 int lvrhip_size(lvrhip_t h, size_t* n)
 {
     return guarded([&] {
@@ -1628,6 +1770,23 @@ int lvrhip_autotune(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float
         REQUIRE(h, "NULL handle");
         DeviceGuard dg(h->device);
         h->autotune(lanes_per_query, tiles_per_wave, r2_margin);
+    });
+}
+
+int lvrhip_set_retry_mode(lvrhip_t h, int merged)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        h->drop_graph();
+        h->merge_retry = merged != 0;
+    });
+}
+
+int lvrhip_get_retry_mode(lvrhip_t h, int* merged)
+{
+    return guarded([&] {
+        REQUIRE(h && merged, "NULL argument");
+        *merged = h->merge_retry ? 1 : 0;
     });
 }
 

@@ -137,6 +137,7 @@ __device__ inline void plane_from_moments(int method, int kcount, float qx, floa
 // itself runs in plane_fit_kernel.  Also the sorted ki-prefix for the interpolation.
 template <int P>
 struct SinkNormals {
+    static constexpr int kTargets = 2;
     // parameters
     float* mom;            // [qpos * 9] moments of the k-set about q
     uint32_t* retry;       // per query slot: 1 if the bbox test failed (NULL in the last round)
@@ -224,6 +225,112 @@ struct SinkNormals {
     }
 };
 
+// The later adaptive rounds in one search (AdaptiveKSearchSurface.tcc:187-216, rounds 2..5):
+// targets t0 = 32 kn (round 5), t1 = 16 kn, t2 = 8 kn, t3 = 4 kn (round 2).  Each round's k-set
+// is a prefix of the next one's, so one selection yields every round's bounding box; the
+// point's adaptive round is the first whose box passes boundingBoxOK, else round 5, as in the
+// round-by-round loop.  Only round 5's moments are accumulated (the registers of four moment
+// sets would halve the occupancy); a point that stops earlier is flagged in `redo` with its
+// round and searched again at that round's k.
+template <int P>
+struct SinkLevels {
+    static constexpr int kTargets = 4;
+    float* mom;       // out [qpos * 9]: moments of the 32 kn-set about q (round-5 points)
+    uint8_t* kround;  // out: the adaptive round
+    uint32_t* redo;   // out [query slot]: the round (2..4) to search again, 0 for round 5
+    uint32_t qi_;
+    float qx_, qy_, qz_;
+    float m[9];
+    float lo[4][3], hi[4][3];
+
+    __device__ __forceinline__ void begin(int, int, int64_t qi, uint32_t, float qx, float qy, float qz)
+    {
+        qi_ = (uint32_t)qi;
+        qx_ = qx;
+        qy_ = qy;
+        qz_ = qz;
+#pragma unroll
+        for (int i = 0; i < 9; i++) m[i] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                lo[t][i] = INFINITY;
+                hi[t][i] = -INFINITY;
+            }
+    }
+
+    __device__ __forceinline__ void accept(int tau, int, int, float x, float y, float z, uint64_t, uint32_t,
+                                           uint32_t*)
+    {
+        lo[tau][0] = fminf(lo[tau][0], x);
+        lo[tau][1] = fminf(lo[tau][1], y);
+        lo[tau][2] = fminf(lo[tau][2], z);
+        hi[tau][0] = fmaxf(hi[tau][0], x);
+        hi[tau][1] = fmaxf(hi[tau][1], y);
+        hi[tau][2] = fmaxf(hi[tau][2], z);
+        if (tau == 0) {
+            const float rx = x - qx_, ry = y - qy_, rz = z - qz_;
+            m[0] += rx;
+            m[1] += ry;
+            m[2] += rz;
+            m[3] += rx * rx;
+            m[4] += rx * ry;
+            m[5] += rx * rz;
+            m[6] += ry * ry;
+            m[7] += ry * rz;
+            m[8] += rz * rz;
+        }
+    }
+
+    __device__ __forceinline__ void finish(int, int s, bool ok, uint32_t qpos, float, float, float, const uint32_t*,
+                                           const uint32_t*)
+    {
+        bool pass[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const float dx = group_max<P>(hi[t][0]) - group_min<P>(lo[t][0]);
+            const float dy = group_max<P>(hi[t][1]) - group_min<P>(lo[t][1]);
+            const float dz = group_max<P>(hi[t][2]) - group_min<P>(lo[t][2]);
+            const float e = 0.05f;
+            pass[t] = dx >= e * dy && dx >= e * dz && dy >= e * dx && dy >= e * dz && dz >= e * dx && dz >= e * dy;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) m[i] = group_sum<P>(m[i]);
+        if (!ok || s != 0) return;
+        // rounds 2, 3, 4 are targets 3, 2, 1; round 5 (target 0) is taken whatever its box
+        const int round = pass[3] ? 2 : pass[2] ? 3 : pass[1] ? 4 : 5;
+        kround[qpos] = (uint8_t)round;
+        redo[qi_] = round < 5 ? (uint32_t)round : 0u;
+        if (round == 5) {
+            float* o = mom + (size_t)qpos * 9;
+#pragma unroll
+            for (int i = 0; i < 9; i++) o[i] = m[i];
+        }
+    }
+};
+
+// Adaptive-round counts of the queries of the merged later rounds: cnt[r] += #(kround >= r),
+// r = 3..5 (cnt[2], the queries themselves, comes from the compaction).
+__global__ __launch_bounds__(256) void round_count_kernel(const uint32_t* __restrict__ qsel,
+                                                          const uint32_t* __restrict__ nq_dev, int64_t nq,
+                                                          const uint8_t* __restrict__ kround, uint32_t* __restrict__ cnt)
+{
+    __shared__ uint32_t c[3];
+    if (threadIdx.x < 3) c[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t n = nq_dev ? (int64_t)*nq_dev : nq;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int r = kround[qsel ? qsel[i] : (uint32_t)i];
+        if (r >= 3) atomicAdd(&c[0], 1u);
+        if (r >= 4) atomicAdd(&c[1], 1u);
+        if (r >= 5) atomicAdd(&c[2], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 && c[threadIdx.x]) atomicAdd(&cnt[3 + threadIdx.x], c[threadIdx.x]);
+}
+
 // Plane fit + flip (AdaptiveKSearchSurface.tcc:225-274) for the queries of one adaptive round
 // whose neighbourhood passed boundingBoxOK (or all of them in the last round).
 struct PlaneFitParams {
@@ -236,7 +343,9 @@ struct PlaneFitParams {
     const float* py;
     const float* pz;
     int method;
-    int kcount;               // |k-set| of this round
+    int kcount;               // |k-set| of this round; < 0: min(kn << kround[q], kmax) per query
+    int kn;
+    int64_t kmax;
     float fx, fy, fz;         // flip point (one scan)
     const float* flips;       // per scan flip points (x, y, z), overrides fx/fy/fz
     const uint16_t* scan_of;  // per sorted position: scan
@@ -259,7 +368,12 @@ __global__ __launch_bounds__(256) void plane_fit_kernel(PlaneFitParams a)
 #pragma unroll
     for (int i = 0; i < 9; i++) m[i] = a.mom[(size_t)qpos * 9 + i];
     float n0, n1, n2;
-    plane_from_moments(a.method, a.kcount, qx, qy, qz, m, n0, n1, n2);
+    int kc = a.kcount;
+    if (kc < 0) {
+        const int64_t k = (int64_t)a.kn << a.kround[qpos];
+        kc = (int)(k < a.kmax ? k : a.kmax);
+    }
+    plane_from_moments(a.method, kc, qx, qy, qz, m, n0, n1, n2);
     float fx = a.fx, fy = a.fy, fz = a.fz;
     if (a.flips) {
         const int sc = a.scan_of ? (int)a.scan_of[qpos] : 0;
@@ -278,11 +392,12 @@ __global__ __launch_bounds__(256) void plane_fit_kernel(PlaneFitParams a)
     a.nx[qpos] = n0;
     a.ny[qpos] = n1;
     a.nz[qpos] = n2;
-    if (a.kround) a.kround[qpos] = (uint8_t)a.round;
+    if (a.kround && a.kcount >= 0) a.kround[qpos] = (uint8_t)a.round;
 }
 
 template <int P>
 struct SinkSdf {
+    static constexpr int kTargets = 2;
     const float* px;
     const float* py;
     const float* pz;
@@ -356,6 +471,7 @@ struct SinkSdf {
 
 template <int P>
 struct SinkList {
+    static constexpr int kTargets = 2;
     uint64_t* out;          // [qorig * kw + slot], unsorted
     const uint32_t* qorig;
     int kw;

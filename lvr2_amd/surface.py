@@ -181,16 +181,22 @@ class HipSurface:
     def setTuning(self, lanes_per_query: int = 0, tiles_per_wave: int = 0, r2_margin: float = 0.0) -> None:
         check(lib().lvrhip_set_tuning(self._h, int(lanes_per_query), int(tiles_per_wave), float(r2_margin)))
 
+    def setRetryMode(self, merged: bool) -> None:
+        """Adaptive rounds 2..5 in one four-target search (True) or round by round (False)."""
+        check(lib().lvrhip_set_retry_mode(self._h, int(bool(merged))))
+
     def autotune(self) -> dict:
-        """Pick lanes per query / tiles per wave / radius margin by timing a sample."""
+        """Pick lanes per query / tiles per wave / radius margin / retry mode by timing a sample."""
         p, t, m = C.c_int(), C.c_int(), C.c_float()
         check(lib().lvrhip_autotune(self._h, C.byref(p), C.byref(t), C.byref(m)))
-        return {"lanes_per_query": p.value, "tiles_per_wave": t.value, "r2_margin": m.value}
+        return self.tuning()
 
     def tuning(self) -> dict:
-        p, t, m = C.c_int(), C.c_int(), C.c_float()
+        p, t, m, r = C.c_int(), C.c_int(), C.c_float(), C.c_int()
         check(lib().lvrhip_get_tuning(self._h, C.byref(p), C.byref(t), C.byref(m)))
-        return {"lanes_per_query": p.value, "tiles_per_wave": t.value, "r2_margin": m.value}
+        check(lib().lvrhip_get_retry_mode(self._h, C.byref(r)))
+        return {"lanes_per_query": p.value, "tiles_per_wave": t.value, "r2_margin": m.value,
+                "merged_retry": bool(r.value)}
 
     def distancesBackward(self, queries, grad_proj, grad_points=None):
         """Reverse mode of ``distances``' proj w.r.t. the points (device tensors only):
@@ -226,6 +232,12 @@ class HipSurface:
 
     def synchronize(self) -> None:
         check(lib().lvrhip_synchronize(self._h))
+
+    def adaptiveK(self) -> np.ndarray:
+        """Per point (input order): the k its normal was fitted on (the adaptive loop's final k)."""
+        out = np.empty(self._n, np.int32)
+        check(lib().lvrhip_get_adaptive_k(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def normalsStats(self) -> np.ndarray:
         """Points searched in each adaptive round (k = kn*2, kn*4, ..., kn*32)."""

@@ -119,7 +119,8 @@ def test_normals_methods_ls_and_iterative(c1, oracle_mod):
         assert np.quantile(a, 0.999) < 1e-4, (method, np.quantile(a, 0.999))
 
 
-def test_adaptive_rounds_match_oracle(oracle_mod):
+@pytest.mark.parametrize("merged", [True, False])
+def test_adaptive_rounds_match_oracle(oracle_mod, merged):
     rng = np.random.default_rng(21)
     # a noise-free axis plane never passes boundingBoxOK; a blob passes at once; a thin
     # strip passes at an intermediate k
@@ -134,10 +135,12 @@ def test_adaptive_rounds_match_oracle(oracle_mod):
     s.setKn(4)
     s.setKi(0)
     s.setFlippoint(0, 0, 100)
+    s.setRetryMode(merged)
     s.calculateNormals()
     stats = s.normalsStats()
     counts = [np.sum(k_or >= 4 * 2 ** (r + 1)) for r in range(5)]
     np.testing.assert_array_equal(stats, counts)
+    np.testing.assert_array_equal(s.adaptiveK(), k_or)
     g = s.getNormals()
     assert np.max(axis_angles(g[:6000], n64[:6000])) < 1e-6
 
@@ -194,7 +197,8 @@ def test_end_to_end_plain_small(c1, oracle_mod):
     assert np.mean(axis_angles(g[stable], oi[stable]) < 1e-3) > 0.999
 
 
-def test_room_subset_vs_oracle(oracle_mod):
+@pytest.mark.parametrize("merged", [True, False])
+def test_room_subset_vs_oracle(oracle_mod, merged):
     from lvr2_amd import scenes
 
     w = scenes.room(1600, 625)  # 1,000,000 rays
@@ -202,11 +206,14 @@ def test_room_subset_vs_oracle(oracle_mod):
     s.setKn(w.kn)
     s.setKi(w.ki)
     s.setFlippoint(*w.flip)
+    s.setRetryMode(merged)
     s.calculateNormals()
     g = s.getNormals()
     o = oracle_mod.Surface(w.points)
     sub = np.arange(0, len(w.points), 97)
     n32, n64, k_or, gap = o.normals(w.kn, 3, w.flip, subset=sub, gap=True)
+    np.testing.assert_array_equal(s.normalsStats()[0], len(w.points))
+    np.testing.assert_array_equal(s.adaptiveK()[sub], k_or)  # the adaptive k of every point
     flipdir = np.asarray(w.flip, np.float64) - w.points[sub].astype(np.float64)
     amb = np.abs(np.sum(n64 * flipdir, 1)) < 1e-4 * np.linalg.norm(flipdir, axis=1)
     good = (gap >= GAP_MIN) & ~amb
