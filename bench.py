@@ -5,7 +5,7 @@ Workload: terrestrial-scan-like room, 4,000,000 float32 points ray-cast from a s
 origin (3200 x 1250 angular grid) into a 20 x 15 x 4 m room with 40 boxes/cylinders, radial
 noise 2 mm; kn=50, ki=20, kd=10, flip point = scanner origin, voxel 0.05 m.  The SDF query
 vertices are the PointsetGrid lattice of that cloud (cells touched by points, extruded by one
-cell, all cell corners).
+cell, all cell corners), built on the GPU by the library.
 
 A step = calculateNormals (adaptive kNN + PCA + flip) + interpolateNormals + distances at all
 query vertices, inputs resident in HBM.  value = points / (t_normals + t_interpolate) in
@@ -31,48 +31,6 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def grid_vertices(points_t, voxel: float, extrude: bool = True):
-    """PointsetGrid vertex set (PointsetGrid.tcc:41-109, HashGrid::addBox) on the GPU with torch:
-    cells floor(p / voxel) whose centre lies in the cloud's box, optionally extruded by the 26
-    neighbours, then every cell corner once, at centre - voxel/2 per axis of its cell."""
-    import torch
-
-    dev = points_t.device
-    vs = torch.tensor(voxel, dtype=torch.float32, device=dev)
-    lo = points_t.min(0).values
-    hi = points_t.max(0).values
-    off = 1 << 20
-
-    def pack(c):
-        c = c + off
-        return (c[:, 0] << 42) | (c[:, 1] << 21) | c[:, 2]
-
-    def unpack(k):
-        return torch.stack([(k >> 42) - off, ((k >> 21) & ((1 << 21) - 1)) - off, (k & ((1 << 21) - 1)) - off], 1)
-
-    def centre_inside(c):
-        ctr = (c.to(torch.float32) + 0.5) * vs
-        return ((ctr >= lo) & (ctr <= hi)).all(1)
-
-    cells = torch.floor(points_t / vs).to(torch.int64)
-    cells = cells[centre_inside(cells)]
-    keys = torch.unique(pack(cells))
-    if extrude:
-        c = unpack(keys)
-        d = torch.stack(torch.meshgrid(*(torch.arange(-1, 2, device=dev),) * 3, indexing="ij"), -1).reshape(-1, 3)
-        allc = (c[:, None, :] + d[None]).reshape(-1, 3)
-        allc = allc[centre_inside(allc)]
-        keys = torch.unique(pack(allc))
-        del allc
-    c = unpack(keys)
-    corners = torch.stack(torch.meshgrid(*(torch.arange(0, 2, device=dev),) * 3, indexing="ij"), -1).reshape(-1, 3)
-    vk = torch.unique(pack((c[:, None, :] + corners[None]).reshape(-1, 3)))
-    lat = unpack(vk)
-    # the corner shared by cells c..c-1 sits at ((c-1) + 0.5) * voxel + voxel/2
-    pos = ((lat - 1).to(torch.float32) + 0.5) * vs + vs / 2
-    return pos.contiguous(), int(keys.numel())
 
 
 def cpu_baseline(w, gpu_normals, verts, sample_pts: int, sample_q: int):
@@ -291,15 +249,21 @@ def main():
     dev = torch.device("cuda:0")
     pts_t = torch.from_numpy(w.points).to(dev)
     st = torch.cuda.current_stream(dev)
-    log(f"[bench] {n} points; building lattice")
-    verts_t, ncells = grid_vertices(pts_t, w.voxel)
-    m = int(verts_t.shape[0])
-    log(f"[bench] {m} query vertices ({ncells} cells); building LBVH")
+    log(f"[bench] {n} points; building LBVH")
     torch.cuda.synchronize()
     t = time.perf_counter()
     s = HipSurface(pts_t)
     torch.cuda.synchronize()
     build_ms = (time.perf_counter() - t) * 1e3
+    s.grid(w.voxel, True, device=True)  # warm-up of the lattice kernels
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    verts_t, _, cell_idx = s.grid(w.voxel, True, device=True)
+    torch.cuda.synchronize()
+    lattice_ms = (time.perf_counter() - t) * 1e3
+    m = int(verts_t.shape[0])
+    ncells = int(cell_idx.shape[0])
+    log(f"[bench] {m} query vertices ({ncells} cells, lattice {lattice_ms:.1f} ms)")
     s.setKn(w.kn)
     s.setKi(w.ki)
     s.setKd(w.kd)
@@ -403,7 +367,7 @@ def main():
                    "kn": w.kn, "ki": w.ki, "kd": w.kd, "voxel": w.voxel},
         "sdf_mqps": round(sdf_mqps, 2),
         "phase_ms": {"normals": round(t_norm, 3), "interpolate": round(t_int, 3), "sdf": round(t_sdf, 3),
-                     "build": round(build_ms, 2)},
+                     "build": round(build_ms, 2), "lattice": round(lattice_ms, 2)},
         "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in tm.items() if v[1]},
         "adaptive_rounds": [int(x) for x in rounds],
         "roofline": roofline,

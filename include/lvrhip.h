@@ -186,6 +186,21 @@ int lvrhip_normals_stats(lvrhip_t h, int64_t n_round[5]);
  * order.  No lvr2 counterpart (the loop's k is local there). */
 int lvrhip_get_adaptive_k(lvrhip_t h, int32_t* out_k);
 
+/* The PointsetGrid lattice on the device (PointsetGrid.tcc:41-109 with HashGrid::addLatticePoints
+ * and addBox, HashGrid.tcc:235-259, 333-378): cells floor(p / voxel) whose centre lies in the
+ * cloud's bounding box, with extrude != 0 also their 26 neighbours (centre in the box), in
+ * increasing (x, y, z) index order; each cell's 8 corners at centre + box_creation_table[v] *
+ * voxel / 2 (FastReconstructionTables.hpp:86-95), a corner shared with an earlier cell reused, so
+ * vertex ids follow first use as in the reference's sequential insertion.  Returns the vertex
+ * and cell counts; the lattice stays in the handle (single-scan handles only).  Cell indices
+ * must lie within +-2^20. */
+int lvrhip_grid_build(lvrhip_t h, float voxel, int extrude, size_t* nv, size_t* nc);
+/* Copies of the last lattice: vertices (nv*3 floats, the positions lvrhip_distances takes),
+ * cells (nc*8 vertex ids per cell, corner order of box_creation_table), cell index (nc*3 int32).
+ * Any output may be NULL.  _device: device pointers, on the handle's stream, not synchronised. */
+int lvrhip_grid_get(lvrhip_t h, float* vertices, int32_t* cells, int32_t* cell_index);
+int lvrhip_grid_get_device(lvrhip_t h, float* d_vertices, int32_t* d_cells, int32_t* d_cell_index);
+
 /* Number of points of the handle. */
 int lvrhip_size(lvrhip_t h, size_t* n);
 
@@ -205,12 +220,13 @@ enum {
 int lvrhip_set_profiling(lvrhip_t h, int on);
 
 /* Kernel work counters collected while profiling is on (no lvr2 counterpart):
- * out[g * 16 + c] for group g (0 first normal round, 1 later rounds, 2 SDF, 3 other kNN) and
- * counter c (0 tiles, 1 radius attempts, 2 LDS-resident attempts, 3 candidate points
- * scanned in the histogram pass, 4 refinement passes, 5 chunk loads, 6 queries resolved,
- * 7 traversal node pops; 8..15 shader-clock cycles summed over waves in: traversal, chunk
- * staging, per-query leaf lists, histogram scan, selection, collection scan, finish, total). */
-int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[64], int reset);
+ * out[g * 16 + c] for group g (0..4 the normals' adaptive rounds 1..5, 5 the merged later
+ * rounds, 6 SDF, 7 other searches: kSearch, the ki list, the adjoint) and counter c (0 tiles,
+ * 1 radius attempts, 2 LDS-resident attempts, 3 candidate points scanned in the histogram
+ * pass, 4 refinement passes, 5 chunk loads, 6 queries resolved, 7 traversal node pops;
+ * 8..15 shader-clock cycles summed over waves in: traversal, chunk staging, per-query leaf
+ * lists, histogram scan, selection, collection scan, finish, total). */
+int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[128], int reset);
 
 /* Launch shape of the kNN tile kernels (no lvr2 counterpart): lanes per query (2, 4 or 8;
  * queries per wavefront = 64 / lanes), consecutive tiles per wavefront, and the radius

@@ -49,11 +49,14 @@ _SIGS = {
     "lvrhip_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "lvrhip_set_retry_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "lvrhip_get_adaptive_k": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lvrhip_grid_build": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "lvrhip_grid_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lvrhip_grid_get_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lvrhip_get_retry_mode": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "lvrhip_distances_backward_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
 }
 
-STAT_GROUPS = ("normals_first", "normals_retry", "sdf", "other")
+STAT_GROUPS = ("round1", "round2", "round3", "round4", "round5", "merged_rounds", "sdf", "other")
 STAT_COUNTERS = ("tiles", "attempts", "resident", "candidates", "refine", "chunks", "resolved", "pops",
                  "cy_trav", "cy_load", "cy_glist", "cy_hist", "cy_select", "cy_collect", "cy_finish", "cy_total")
 

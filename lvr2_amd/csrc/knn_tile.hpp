@@ -430,6 +430,18 @@ struct Target {
     bool refine;
 };
 
+// Sinks with an `lds` member get the candidate staging area, free once a query's collection
+// and ranking are done (their finish may use it).
+template <class S>
+__device__ __forceinline__ auto give_lds(S& s, float4* p, int) -> decltype(s.lds = p, void())
+{
+    s.lds = p;
+}
+template <class S>
+__device__ __forceinline__ void give_lds(S&, float4*, long)
+{
+}
+
 template <int P, class Sink>
 // LDS allows about two of these waves per SIMD: the register budget is set for that occupancy.
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) void knn_tile_kernel(
@@ -447,6 +459,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
     ws.clear();
     ws.timing = prm.stats != nullptr;
     const auto t_wave = ws.tick();
+    give_lds(sink, sm.cand, 0);
     const uint32_t nq = qv.nq_dev ? uniform(*qv.nq_dev) : qv.nq;
     if (prm.area_pp) r2 = fmaxf(uniformf(*prm.area_pp) * (float)prm.t0 / 3.14159265f * prm.r2_margin, 1e-30f);
     const int64_t ntiles = ((int64_t)nq + Q - 1) / Q;

@@ -7,6 +7,7 @@
 #include "knn_tile.hpp"
 #include "sinks.hpp"
 #include "adjoint.hpp"
+#include "lattice.hpp"
 
 #include "lvrhip.h"
 
@@ -416,7 +417,10 @@ struct lvrhip_surface {
     int64_t n_round[5] = {0, 0, 0, 0, 0};
     DBuf<int> err;
     Profiler prof;
-    DBuf<unsigned long long> kstats;  // 4 groups x KS_COUNT
+    // work counters, KS_GROUPS groups x KS_COUNT: normals rounds 1..5, merged later rounds,
+    // SDF, other searches (kSearch, the ki list, the adjoint)
+    static constexpr int KS_GROUPS = 8;
+    DBuf<unsigned long long> kstats;
 
     hipStream_t gst = nullptr;  // private stream for graphs when the handle's stream is the null stream
     bool own_gst = false;
@@ -435,8 +439,8 @@ struct lvrhip_surface {
     {
         if (!prof.on) return nullptr;
         if (!kstats.p) {
-            kstats.reserve(4 * KS_COUNT);
-            LVRHIP_CHECK(hipMemsetAsync(kstats.p, 0, 4 * KS_COUNT * sizeof(unsigned long long), st));
+            kstats.reserve(KS_GROUPS * KS_COUNT);
+            LVRHIP_CHECK(hipMemsetAsync(kstats.p, 0, KS_GROUPS * KS_COUNT * sizeof(unsigned long long), st));
         }
         return kstats.p + group * KS_COUNT;
     }
@@ -682,7 +686,7 @@ struct lvrhip_surface {
         for (int r = 1; r <= last; r++) {
             const int t0 = clamp_k((int64_t)kn << r);
             TileParams prm = params(t0, r == 1 ? kiw : 0, 0);
-            prm.stats = kstats_ptr(r == 1 ? 0 : 1);
+            prm.stats = kstats_ptr(r - 1);
             const uint32_t* qsel = r == 1 ? q0 : lists[(r - 1) & 1];
             const uint32_t* nq_dev = r == 1 ? nullptr : d_cnt.p + r;
             QueryView qv{px.p, py.p, pz.p, qsel, (uint32_t)nq0, nq_dev};
@@ -736,7 +740,7 @@ struct lvrhip_surface {
         TileParams prm = params(clamp_k((int64_t)kn << 5), clamp_k((int64_t)kn << 4), 1);
         prm.t2 = clamp_k((int64_t)kn << 3);
         prm.t3 = clamp_k((int64_t)kn << 2);
-        prm.stats = kstats_ptr(1);
+        prm.stats = kstats_ptr(5);
         uint32_t* redo = s_flags.p;
         {
             QueryView qv{px.p, py.p, pz.p, qsel, (uint32_t)nq0, nq_dev};
@@ -765,7 +769,7 @@ struct lvrhip_surface {
             compact_flags(a_sel.p, nq0, nq_dev, qsel, a_list.p, d_cnt.p + 6);
             const int t0 = clamp_k((int64_t)kn << r);
             TileParams pr = params(t0, 0, 1);
-            pr.stats = kstats_ptr(1);
+            pr.stats = kstats_ptr(r - 1);
             QueryView qv{px.p, py.p, pz.p, a_list.p, (uint32_t)nq0, d_cnt.p + 6};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
@@ -840,7 +844,7 @@ struct lvrhip_surface {
         kiw = kw;
         ki_list.reserve((size_t)npad * kw);
         TileParams prm = params(kw, 0, 3);
-        prm.stats = kstats_ptr(3);
+        prm.stats = kstats_ptr(7);
         QueryView qv{px.p, py.p, pz.p, all_queries(), (uint32_t)n, nullptr};
         auto mk = [&](auto tag) {
             constexpr int P = decltype(tag)::value;
@@ -1128,7 +1132,7 @@ struct lvrhip_surface {
         sort_queries(d_q, m);
         prof.stop(tsort, st);
         TileParams prm = params(t0, 0, 2);
-        prm.stats = kstats_ptr(2);
+        prm.stats = kstats_ptr(6);
         QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
         auto mk = [&](auto tag) {
             constexpr int P = decltype(tag)::value;
@@ -1157,6 +1161,138 @@ struct lvrhip_surface {
         prof.stop(tok, st);
         check_err();
         prof.collect();
+    }
+
+    // ---- the PointsetGrid lattice (lattice.hpp) -------------------------------------------------
+    DBuf<uint64_t> g_k0, g_k1, g_cells, g_cells2, g_first;
+    DBuf<uint32_t> g_v0, g_v1, g_flags, g_idx, g_gid, g_vidx, g_vid, g_hist;
+    DBuf<float> grid_vtx;
+    DBuf<int32_t> grid_cells, grid_cidx;
+    int64_t grid_nv = 0, grid_nc = 0;
+
+    void lat_sort(uint64_t* k, uint32_t* v, int64_t m, int end_bit = 64)
+    {
+        g_hist.reserve((size_t)RS_RADIX * ceil_div(std::max<int64_t>(m, 1), RS_TILE));
+        radix_sort_pairs(k, v, g_k1.p, g_v1.p, m, g_hist.p, st, end_bit);
+    }
+
+    // unique non-sentinel keys of the sorted keys[0, m) into out; returns their count
+    int64_t lat_unique(const uint64_t* keys, int64_t m, uint64_t* out)
+    {
+        g_flags.reserve(m);
+        g_idx.reserve(m);
+        lat_first_flags_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(keys, m, g_flags.p);
+        LVRHIP_LAUNCH_CHECK();
+        compact_flags(g_flags.p, m, nullptr, nullptr, g_idx.p, d_cnt.p + 7);
+        lat_gather_keys_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(keys, g_idx.p, d_cnt.p + 7, out);
+        LVRHIP_LAUNCH_CHECK();
+        uint32_t c = 0;
+        LVRHIP_CHECK(hipMemcpyAsync(&c, d_cnt.p + 7, sizeof(c), hipMemcpyDeviceToHost, st));
+        LVRHIP_CHECK(hipStreamSynchronize(st));
+        return c;
+    }
+
+    void build_grid(float voxel, bool extrude)
+    {
+        require_one_scan("the lattice");
+        if (!(voxel > 0.0f) || std::isinf(voxel)) throw Error(LVRHIP_EINVAL, "voxel size must be positive");
+        drop_graph();
+        grid_nv = grid_nc = 0;
+        if (n == 0) return;
+        d_cnt.reserve(8);
+        // the cloud's box (the surface's bounding box, PointsetSurface.tcc:52-56)
+        uint32_t enc[6];
+        LVRHIP_CHECK(hipMemcpyAsync(enc, sbox.p, sizeof(enc), hipMemcpyDeviceToHost, st));
+        LVRHIP_CHECK(hipStreamSynchronize(st));
+        LatBox b;
+        for (int j = 0; j < 6; j++) {
+            const uint32_t e = enc[j];
+            uint32_t bits = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
+            float f;
+            std::memcpy(&f, &bits, 4);
+            (j < 3 ? b.lo[j] : b.hi[j - 3]) = f;
+        }
+        b.voxel = voxel;
+        for (int j = 0; j < 3; j++) {
+            const double a = std::floor((double)b.lo[j] / voxel), c = std::floor((double)b.hi[j] / voxel);
+            if (!(a > -(double)(LAT_BIAS - 2) && c < (double)(LAT_BIAS - 2)))
+                throw Error(LVRHIP_EINVAL, "lattice index out of range: voxel too small for the cloud's extent");
+        }
+        const int64_t R = npad;
+        int64_t m = R;
+        g_k0.reserve(m);
+        g_k1.reserve(m);
+        g_v0.reserve(m);
+        g_v1.reserve(m);
+        lat_point_cells_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(px.p, py.p, pz.p, pidx.p, R, b, g_k0.p,
+                                                                          g_v0.p);
+        LVRHIP_LAUNCH_CHECK();
+        lat_sort(g_k0.p, g_v0.p, m);
+        g_cells.reserve(m);
+        int64_t nc = lat_unique(g_k0.p, m, g_cells.p);
+        if (extrude && nc > 0) {
+            m = 27 * nc;
+            g_k0.reserve(m);
+            g_k1.reserve(m);
+            g_v0.reserve(m);
+            g_v1.reserve(m);
+            lat_extrude_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(g_cells.p, nc, b, g_k0.p, g_v0.p);
+            LVRHIP_LAUNCH_CHECK();
+            lat_sort(g_k0.p, g_v0.p, m);
+            g_cells2.reserve(m);
+            nc = lat_unique(g_k0.p, m, g_cells2.p);
+            std::swap(g_cells.p, g_cells2.p);
+            std::swap(g_cells.cap, g_cells2.cap);
+        }
+        if (nc == 0) return;
+        if (8 * nc >= (int64_t)0xFFFFFFFFu) throw Error(LVRHIP_ECAPACITY, "lattice too large");
+        // corners: sorted by corner key, stable in order = cell rank * 8 + slot
+        m = 8 * nc;
+        g_k0.reserve(m);
+        g_k1.reserve(m);
+        g_v0.reserve(m);
+        g_v1.reserve(m);
+        lat_corners_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(g_cells.p, nc, g_k0.p, g_v0.p);
+        LVRHIP_LAUNCH_CHECK();
+        lat_sort(g_k0.p, g_v0.p, m);
+        g_flags.reserve(m);
+        lat_first_flags_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(g_k0.p, m, g_flags.p);
+        LVRHIP_LAUNCH_CHECK();
+        const int64_t nb = ceil_div(m, (int64_t)LAT_TILE);
+        s_bsum.reserve(nb + 1);
+        flag_block_sums_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(g_flags.p, m, nullptr, s_bsum.p);
+        LVRHIP_LAUNCH_CHECK();
+        scan_exclusive_kernel<<<1, 1024, 0, st>>>(s_bsum.p, nb, d_cnt.p + 7);
+        LVRHIP_LAUNCH_CHECK();
+        g_gid.reserve(m);
+        lat_group_ids_kernel<<<(unsigned)nb, LAT_THREADS, 0, st>>>(g_flags.p, m, s_bsum.p, g_gid.p);
+        LVRHIP_LAUNCH_CHECK();
+        uint32_t nvu = 0;
+        LVRHIP_CHECK(hipMemcpyAsync(&nvu, d_cnt.p + 7, sizeof(nvu), hipMemcpyDeviceToHost, st));
+        LVRHIP_CHECK(hipStreamSynchronize(st));
+        const int64_t nv = nvu;
+        // vertex ids in first-use order
+        g_first.reserve(nv);
+        g_vidx.reserve(nv);
+        lat_first_order_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(g_flags.p, g_gid.p, g_v0.p, m, g_first.p,
+                                                                          g_vidx.p);
+        LVRHIP_LAUNCH_CHECK();
+        lat_sort(g_first.p, g_vidx.p, nv, 32);
+        g_vid.reserve(nv);
+        lat_vertex_ids_kernel<<<(unsigned)ceil_div(nv, 256), 256, 0, st>>>(g_vidx.p, nv, g_vid.p);
+        LVRHIP_LAUNCH_CHECK();
+        grid_cells.reserve(m);
+        lat_cells_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(g_gid.p, g_v0.p, g_vid.p, m, grid_cells.p);
+        LVRHIP_LAUNCH_CHECK();
+        grid_vtx.reserve(3 * nv);
+        lat_vertices_kernel<<<(unsigned)ceil_div(nv, 256), 256, 0, st>>>(g_first.p, g_cells.p, nv, voxel, grid_vtx.p);
+        LVRHIP_LAUNCH_CHECK();
+        grid_cidx.reserve(3 * nc);
+        lat_cell_index_kernel<<<(unsigned)ceil_div(nc, 256), 256, 0, st>>>(g_cells.p, nc, grid_cidx.p);
+        LVRHIP_LAUNCH_CHECK();
+        LVRHIP_CHECK(hipStreamSynchronize(st));
+        grid_nv = nv;
+        grid_nc = nc;
     }
 
     // Reverse pass of distances: d_grad (n x 3, input order) += d(sum_v w_v proj_v)/d(points)
@@ -1189,7 +1325,7 @@ struct lvrhip_surface {
         // 1. SDF reverse: dL/dp (through the means) and dL/dn of every vertex's kd neighbours
         {
             TileParams prm = params(t0, 0, 2);
-            prm.stats = kstats_ptr(2);
+            prm.stats = kstats_ptr(6);
             QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
@@ -1229,7 +1365,7 @@ struct lvrhip_surface {
             LVRHIP_LAUNCH_CHECK();
             compact_flags(a_sel.p, R, nullptr, nullptr, a_list.p, d_cnt.p + 6);
             TileParams prm = params(clamp_k((int64_t)kn << r), 0, 3);
-            prm.stats = kstats_ptr(3);
+            prm.stats = kstats_ptr(7);
             QueryView qv{px.p, py.p, pz.p, a_list.p, (uint32_t)R, d_cnt.p + 6};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
@@ -1261,7 +1397,7 @@ struct lvrhip_surface {
             DBuf<uint64_t> keys;
             keys.reserve((size_t)m * k);
             TileParams prm = params(t0, 0, 3);
-            prm.stats = kstats_ptr(3);
+            prm.stats = kstats_ptr(7);
             QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
@@ -1623,6 +1759,52 @@ int lvrhip_get_adaptive_k(lvrhip_t h, int32_t* out_k)
     });
 }
 
+int lvrhip_grid_build(lvrhip_t h, float voxel, int extrude, size_t* out_nv, size_t* out_nc)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        DeviceGuard dg(h->device);
+        h->build_grid(voxel, extrude != 0);
+        if (out_nv) *out_nv = (size_t)h->grid_nv;
+        if (out_nc) *out_nc = (size_t)h->grid_nc;
+    });
+}
+
+int lvrhip_grid_get_device(lvrhip_t h, float* d_vertices, int32_t* d_cells, int32_t* d_cell_index)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        DeviceGuard dg(h->device);
+        if (d_vertices && h->grid_nv)
+            LVRHIP_CHECK(hipMemcpyAsync(d_vertices, h->grid_vtx.p, 3 * (size_t)h->grid_nv * sizeof(float),
+                                        hipMemcpyDeviceToDevice, h->st));
+        if (d_cells && h->grid_nc)
+            LVRHIP_CHECK(hipMemcpyAsync(d_cells, h->grid_cells.p, 8 * (size_t)h->grid_nc * sizeof(int32_t),
+                                        hipMemcpyDeviceToDevice, h->st));
+        if (d_cell_index && h->grid_nc)
+            LVRHIP_CHECK(hipMemcpyAsync(d_cell_index, h->grid_cidx.p, 3 * (size_t)h->grid_nc * sizeof(int32_t),
+                                        hipMemcpyDeviceToDevice, h->st));
+    });
+}
+
+int lvrhip_grid_get(lvrhip_t h, float* vertices, int32_t* cells, int32_t* cell_index)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        DeviceGuard dg(h->device);
+        if (vertices && h->grid_nv)
+            LVRHIP_CHECK(hipMemcpyAsync(vertices, h->grid_vtx.p, 3 * (size_t)h->grid_nv * sizeof(float),
+                                        hipMemcpyDeviceToHost, h->st));
+        if (cells && h->grid_nc)
+            LVRHIP_CHECK(hipMemcpyAsync(cells, h->grid_cells.p, 8 * (size_t)h->grid_nc * sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, h->st));
+        if (cell_index && h->grid_nc)
+            LVRHIP_CHECK(hipMemcpyAsync(cell_index, h->grid_cidx.p, 3 * (size_t)h->grid_nc * sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, h->st));
+        LVRHIP_CHECK(hipStreamSynchronize(h->st));
+    });
+}
+
 int lvrhip_size(lvrhip_t h, size_t* n)
 {
     return guarded([&] {
@@ -1641,12 +1823,12 @@ int lvrhip_set_profiling(lvrhip_t h, int on)
     });
 }
 
-int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[64], int reset)
+int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[128], int reset)
 {
     return guarded([&] {
         REQUIRE(h && out, "NULL argument");
         DeviceGuard dg(h->device);
-        unsigned long long v[4 * KS_COUNT] = {};
+        unsigned long long v[lvrhip_surface::KS_GROUPS * KS_COUNT] = {};
         if (h->kstats.p) {
             LVRHIP_CHECK(hipMemcpyAsync(v, h->kstats.p, sizeof(v), hipMemcpyDeviceToHost, h->st));
             LVRHIP_CHECK(hipStreamSynchronize(h->st));
@@ -1655,7 +1837,7 @@ int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[64], int reset)
                 LVRHIP_CHECK(hipStreamSynchronize(h->st));
             }
         }
-        for (int i = 0; i < 4 * KS_COUNT; i++) out[i] = (int64_t)v[i];
+        for (int i = 0; i < lvrhip_surface::KS_GROUPS * KS_COUNT; i++) out[i] = (int64_t)v[i];
     });
 }
 

@@ -415,6 +415,7 @@ struct SinkSdf {
     uint8_t* out_invalid;
     uint32_t npts;
     uint32_t qpos_;
+    float4* lds;  // the kernel's candidate staging area (CAP float4), free in finish
 
     __device__ __forceinline__ void begin(int g, int s, int64_t qi, uint32_t qpos, float, float, float) { qpos_ = qpos; }
 
@@ -428,17 +429,76 @@ struct SinkSdf {
         }
     }
 
-    __device__ __forceinline__ void finish(int g, int s, bool ok, uint32_t qpos, float qx, float qy, float qz, const uint32_t* cnt0,
-                           const uint32_t*)
+    // means of the ordered sums, signed distance, invalid flag (tcc:435-457, PointsetGrid.tcc:138)
+    __device__ __forceinline__ void emit(float ax, float ay, float az, float bx, float by, float bz, int cnt,
+                                         uint32_t qpos, float qx, float qy, float qz)
     {
+        const float c = (float)cnt;
+        bx /= c;
+        by /= c;
+        bz /= c;
+        ax /= c;
+        ay /= c;
+        az /= c;
+        normalize3(bx, by, bz);
+        const float dx = qx - ax, dy = qy - ay, dz = qz - az;
+        const float proj = dx * bx + dy * by + dz * bz;
+        const float eucl = sqrtf(dx * dx + dy * dy + dz * dz);
+        const uint32_t oi = qorig ? qorig[qpos] : qpos;
+        if (out_proj) out_proj[oi] = proj;
+        if (out_eucl) out_eucl[oi] = eucl;
+        // the comparison is made in double
+        if (out_invalid) out_invalid[oi] = ((double)eucl > 1.75 * (double)voxel) ? 1 : 0;
+    }
+
+    __device__ __forceinline__ void finish(int g, int s, bool ok, uint32_t qpos, float qx, float qy, float qz,
+                                           const uint32_t* cnt0, const uint32_t*)
+    {
+        constexpr int Q = kWave / P;
+        constexpr int SLOT = CAP * 16 / Q;  // staging bytes per query
+        constexpr int ENT = SLOT / 32;      // neighbours per query: an 8-byte key + 6 floats each
         const int cnt = min((int)cnt0[g], kw);
-        if (ok)
-            group_rank_sort<P>(s, cnt, keys + (size_t)qpos * kw, spos + (size_t)qpos * kw, ord + (size_t)qpos * kw);
-        __syncthreads();
-        if (ok && s == 0) {
-            // ordered fp32 sums, AdaptiveKSearchSurface.tcc:435-449
-            float ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
-            const uint32_t* o = ord + (size_t)qpos * kw;
+        const size_t row = (size_t)qpos * kw;
+        // ordered fp32 sums (AdaptiveKSearchSurface.tcc:435-449) in ascending key order
+        float ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+        if (kw <= ENT && lds) {
+            // rank in LDS and gather the neighbours' points and normals with all P lanes
+            char* base = reinterpret_cast<char*>(lds) + g * SLOT;
+            uint64_t* kk = reinterpret_cast<uint64_t*>(base);
+            float* d = reinterpret_cast<float*>(base + 8 * ENT);
+            if (ok)
+                for (int j = s; j < cnt; j += P) kk[j] = keys[row + j];
+            __syncthreads();
+            if (ok)
+                for (int j = s; j < cnt; j += P) {
+                    const uint64_t kj = kk[j];
+                    int rank = 0;
+                    for (int i = 0; i < cnt; i++) rank += kk[i] < kj ? 1 : 0;
+                    const uint32_t sp = spos[row + j];
+                    const bool in = sp < npts;  // (always, for keys of staged points)
+                    float* o = d + 6 * rank;
+                    o[0] = in ? px[sp] : 0.0f;
+                    o[1] = in ? py[sp] : 0.0f;
+                    o[2] = in ? pz[sp] : 0.0f;
+                    o[3] = in ? nx[sp] : 0.0f;
+                    o[4] = in ? ny[sp] : 0.0f;
+                    o[5] = in ? nz[sp] : 0.0f;
+                }
+            __syncthreads();
+            if (!ok || s != 0) return;
+            for (int j = 0; j < cnt; j++) {
+                ax += d[6 * j];
+                ay += d[6 * j + 1];
+                az += d[6 * j + 2];
+                bx += d[6 * j + 3];
+                by += d[6 * j + 4];
+                bz += d[6 * j + 5];
+            }
+        } else {
+            if (ok) group_rank_sort<P>(s, cnt, keys + row, spos + row, ord + row);
+            __syncthreads();
+            if (!ok || s != 0) return;
+            const uint32_t* o = ord + row;
             for (int j = 0; j < cnt; j++) {
                 const uint32_t sp = o[j];
                 if (sp >= npts) continue;
@@ -449,23 +509,8 @@ struct SinkSdf {
                 by += ny[sp];
                 bz += nz[sp];
             }
-            const float c = (float)cnt;
-            bx /= c;
-            by /= c;
-            bz /= c;
-            ax /= c;
-            ay /= c;
-            az /= c;
-            normalize3(bx, by, bz);
-            const float dx = qx - ax, dy = qy - ay, dz = qz - az;
-            const float proj = dx * bx + dy * by + dz * bz;
-            const float eucl = sqrtf(dx * dx + dy * dy + dz * dz);
-            const uint32_t oi = qorig ? qorig[qpos] : qpos;
-            if (out_proj) out_proj[oi] = proj;
-            if (out_eucl) out_eucl[oi] = eucl;
-            // PointsetGrid.tcc:138: the comparison is made in double
-            if (out_invalid) out_invalid[oi] = ((double)eucl > 1.75 * (double)voxel) ? 1 : 0;
         }
+        emit(ax, ay, az, bx, by, bz, cnt, qpos, qx, qy, qz);
     }
 };
 

@@ -174,7 +174,7 @@ class HipSurface:
         """Work counters of the kNN tile kernels, per group (collected while profiling)."""
         from ._lib import STAT_COUNTERS, STAT_GROUPS
 
-        a = np.zeros(64, np.int64)
+        a = np.zeros(16 * len(STAT_GROUPS), np.int64)
         check(lib().lvrhip_get_kernel_stats(self._h, a.ctypes.data_as(C.c_void_p), int(bool(reset))))
         return {g: dict(zip(STAT_COUNTERS, (int(x) for x in a[16 * i: 16 * i + 16]))) for i, g in enumerate(STAT_GROUPS)}
 
@@ -232,6 +232,30 @@ class HipSurface:
 
     def synchronize(self) -> None:
         check(lib().lvrhip_synchronize(self._h))
+
+    def grid(self, voxel: float, extrude: bool = True, device: bool = False):
+        """The PointsetGrid lattice: (vertices (nv, 3) float32, cells (nc, 8) int32 vertex ids,
+        cell index (nc, 3) int32); torch device tensors with device=True."""
+        nv, nc = C.c_size_t(), C.c_size_t()
+        check(lib().lvrhip_grid_build(self._h, float(voxel), int(bool(extrude)), C.byref(nv), C.byref(nc)))
+        nv, nc = nv.value, nc.value
+        if device:
+            import torch
+
+            dev = torch.device("cuda", self.device)
+            v = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+            c = torch.empty((nc, 8), dtype=torch.int32, device=dev)
+            ci = torch.empty((nc, 3), dtype=torch.int32, device=dev)
+            check(lib().lvrhip_grid_get_device(self._h, C.c_void_p(v.data_ptr()), C.c_void_p(c.data_ptr()),
+                                               C.c_void_p(ci.data_ptr())))
+            torch.cuda.synchronize(dev)
+            return v, c, ci
+        v = np.empty((nv, 3), np.float32)
+        c = np.empty((nc, 8), np.int32)
+        ci = np.empty((nc, 3), np.int32)
+        check(lib().lvrhip_grid_get(self._h, v.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p),
+                                    ci.ctypes.data_as(C.c_void_p)))
+        return v, c, ci
 
     def adaptiveK(self) -> np.ndarray:
         """Per point (input order): the k its normal was fitted on (the adaptive loop's final k)."""

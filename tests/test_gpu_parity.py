@@ -287,3 +287,38 @@ def test_error_paths():
     assert e.value.code == 1
     with pytest.raises(LvrHipError):
         s.setMethod(1)  # RANSAC is not provided
+
+
+@pytest.mark.parametrize("extrude", [True, False])
+def test_lattice_bitexact_plain_small(c1, oracle_mod, extrude):
+    w, o = c1[0], c1[1]
+    s = _surface(w.points)
+    v, cells, ci = s.grid(w.voxel, extrude)
+    ov, ocells, oci = oracle_mod.grid(w.points, o.bbox(), w.voxel, extrude=extrude)
+    np.testing.assert_array_equal(ci, oci)
+    np.testing.assert_array_equal(cells, ocells)
+    np.testing.assert_array_equal(v, ov)
+
+
+def test_lattice_bitexact_room_and_negative_coordinates(oracle_mod):
+    from lvr2_amd import scenes
+
+    w = scenes.room(1600, 625)
+    s = _surface(w.points)
+    v, cells, ci = s.grid(w.voxel, True)
+    ov, ocells, oci = oracle_mod.grid(w.points, oracle_mod.Surface(w.points).bbox(), w.voxel, extrude=True)
+    np.testing.assert_array_equal(ci, oci)
+    np.testing.assert_array_equal(cells, ocells)
+    np.testing.assert_array_equal(v, ov)
+    # a tiny cloud around the origin, a voxel that does not divide the coordinates evenly
+    p = np.random.default_rng(3).normal(scale=0.7, size=(500, 3)).astype(np.float32)
+    s = _surface(p)
+    v, cells, ci = s.grid(0.13, True)
+    ov, ocells, oci = oracle_mod.grid(p, oracle_mod.Surface(p).bbox(), 0.13, extrude=True)
+    np.testing.assert_array_equal(ci, oci)
+    np.testing.assert_array_equal(cells, ocells)
+    np.testing.assert_array_equal(v, ov)
+    # device copies equal the host copies
+    vd, cd, cid = s.grid(0.13, True, device=True)
+    np.testing.assert_array_equal(vd.cpu().numpy(), ov)
+    np.testing.assert_array_equal(cd.cpu().numpy(), ocells)
