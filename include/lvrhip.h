@@ -242,6 +242,11 @@ int lvrhip_get_timings(lvrhip_t h, double* ms, int64_t* launches, int reset);
  * two retry modes (lvrhip_set_retry_mode) at that shape.  Invalidates the normals. */
 int lvrhip_autotune(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float* r2_margin);
 
+/* Launch shape of the signed-distance searches (distances and its reverse pass): consecutive
+ * tiles per wavefront (default 2) and radius margin (default 1.3); lanes per query are shared
+ * with lvrhip_set_tuning.  Values <= 0 keep the current setting.  No lvr2 counterpart. */
+int lvrhip_set_sdf_tuning(lvrhip_t h, int tiles_per_wave, float r2_margin);
+
 /* How calculate_normals runs the adaptive rounds 2..5 for the points round 1 rejected (no lvr2
  * counterpart; the results are the same either way): merged != 0 searches k = 32 kn once with
  * the four round sizes as targets, takes the first round whose neighbourhood passes

@@ -48,6 +48,7 @@ _SIGS = {
     "lvrhip_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "lvrhip_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "lvrhip_set_retry_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "lvrhip_set_sdf_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
     "lvrhip_get_adaptive_k": (C.c_int, [C.c_void_p, C.c_void_p]),
     "lvrhip_grid_build": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "lvrhip_grid_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -31,6 +31,7 @@
 namespace lvrhip {
 
 constexpr int NB = 32;         // histogram bins
+constexpr int NH = 1;          // histograms per query (lane s counts into number s % NH)
 constexpr int CAP = 512;       // candidate slots per chunk
 constexpr int CHUNK_LEAVES = CAP / LEAF;
 constexpr int TL_CAP = 512;    // leaves kept per tile
@@ -72,6 +73,9 @@ struct TileParams {
     float r2_margin;            // next tile's radius^2 = margin * largest k-th d^2 of this tile
     int* err;                   // set to a nonzero code on capacity overflow
     unsigned long long* stats;  // optional counters, see KS_* below
+    uint32_t* queue;            // optional work queue (zeroed before the launch): waves take the
+                                // next run of tiles_per_wave tiles from it; else runs are
+                                // strided over the grid
 };
 
 enum {
@@ -124,7 +128,7 @@ template <int Q, int BUF, int NT>
 struct TileSmem {
     float4 cand[CAP];  // x, y, z, original index (as bits)
     float4 lbox[CHUNK_LEAVES][2];  // boxes of the chunk's leaves (lo, hi)
-    uint32_t hist[Q][NB];
+    uint32_t hist[Q][NH * NB];
     int stack[STACK_CAP];
     int tleaf[TL_CAP];                // the tile's leaves (or the current chunk's, when streaming)
     uint8_t glist[Q][CHUNK_LEAVES];   // per query: chunk leaf slots within its bound
@@ -256,9 +260,12 @@ __device__ __forceinline__ uint32_t slot_spos(const Sm& sm, int base, int c)
 // Per query group g: the chunk's leaves (sm.tleaf[base, base+nl)) whose box lies within
 // sqrt(rq) of the query point, as chunk slots in sm.glist[g]; returns their count
 // (replicated over the group's lanes).  rq < 0 disables the query.
+// With rin >= 0, leaves whose farthest corner is closer than sqrt(rin) are left out too:
+// every point of such a leaf has d^2 < rin (the fp32 corner distance bounds the fp32 point
+// distance from above, all roundings being monotone).
 template <int P, class Sm>
-__device__ __forceinline__ int build_glist(Sm& sm, const BvhView& bv, int base, int nl, int g, int s, float qx, float qy, float qz,
-                           float rq)
+__device__ __forceinline__ int build_glist(Sm& sm, const BvhView& bv, int base, int nl, int g, int s, float qx,
+                                           float qy, float qz, float rq, float rin = -1.0f)
 {
     constexpr uint64_t GM = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
     int len = 0;
@@ -269,6 +276,12 @@ __device__ __forceinline__ int build_glist(Sm& sm, const BvhView& bv, int base, 
             const int leaf = sm.tleaf[base + j];
             const float4 lo = sm.lbox[j][0], hi = sm.lbox[j][1];
             hit = box_d2(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, qx, qy, qz, qx, qy, qz) <= rq;
+            if (hit && rin > 0.0f) {
+                const float fx = fmaxf(fabsf(qx - lo.x), fabsf(qx - hi.x));
+                const float fy = fmaxf(fabsf(qy - lo.y), fabsf(qy - hi.y));
+                const float fz = fmaxf(fabsf(qz - lo.z), fabsf(qz - hi.z));
+                hit = !((fx * fx + fy * fy) + fz * fz < rin);
+            }
         }
         const uint32_t gm = (uint32_t)((ballot(hit) >> (g * P)) & GM);
         if (hit) sm.glist[g][len + __popc(gm & ((1u << s) - 1u))] = (uint8_t)j;
@@ -359,15 +372,23 @@ __device__ __forceinline__ void scan_glist(const Sm& sm, int g, int s, int len, 
 
 // Locate the bin holding the t-th smallest counted key of query group g (P lanes).
 // Returns total count; if t <= total: bin, below (count in lower bins), cnt (in that bin).
+__device__ __forceinline__ int hist_bin(const uint32_t (*hist)[NH * NB], int g, int b)
+{
+    int v = 0;
+#pragma unroll
+    for (int h = 0; h < NH; h++) v += (int)hist[g][h * NB + b];
+    return v;
+}
+
 template <int P, int Q>
-__device__ __forceinline__ int find_bin(const uint32_t (*hist)[NB], int g, int s, int t, int& bin, int& below,
+__device__ __forceinline__ int find_bin(const uint32_t (*hist)[NH * NB], int g, int s, int t, int& bin, int& below,
                                         int& cnt)
 {
     constexpr int BPL = NB / P;
     static_assert(NB % P == 0, "bins per lane");
     int sum = 0;
 #pragma unroll
-    for (int j = 0; j < BPL; j++) sum += (int)hist[g][s * BPL + j];
+    for (int j = 0; j < BPL; j++) sum += hist_bin(hist, g, s * BPL + j);
     int incl = sum;
 #pragma unroll
     for (int o = 1; o < P; o <<= 1) {
@@ -383,7 +404,7 @@ __device__ __forceinline__ int find_bin(const uint32_t (*hist)[NB], int g, int s
 #pragma unroll
         for (int j = 0; j < BPL; j++) {
             const int b = s * BPL + j;
-            const int v = (int)hist[g][b];
+            const int v = hist_bin(hist, g, b);
             if (run < t && t <= run + v) {
                 b_ = b;
                 below_ = run;
@@ -462,11 +483,25 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
     give_lds(sink, sm.cand, 0);
     const uint32_t nq = qv.nq_dev ? uniform(*qv.nq_dev) : qv.nq;
     if (prm.area_pp) r2 = fmaxf(uniformf(*prm.area_pp) * (float)prm.t0 / 3.14159265f * prm.r2_margin, 1e-30f);
+    const float r2_first = r2;
     const int64_t ntiles = ((int64_t)nq + Q - 1) / Q;
 
-    // runs of tiles_per_wave consecutive tiles, strided over the grid (a grid sized for an
-    // upper bound of the device-side count leaves the surplus waves idle)
-    for (int64_t run = blockIdx.x; run * prm.tiles_per_wave < ntiles; run += gridDim.x)
+    // runs of tiles_per_wave consecutive tiles, taken from the work queue (every wave ends once
+    // it is drained) or strided over the grid (a grid sized for an upper bound of the
+    // device-side count leaves the surplus waves idle)
+    int64_t run = -1;
+    for (;;) {
+        if (prm.queue) {
+            uint32_t r = 0;
+            if (lane == 0) r = atomicAdd(prm.queue, 1u);
+            run = (int64_t)(uint32_t)uniform(shfl((int)r, 0));
+        } else {
+            run = run < 0 ? (int64_t)blockIdx.x : run + gridDim.x;
+        }
+        if (run * prm.tiles_per_wave >= ntiles) break;
+        // every run starts from the same radius guess: results do not depend on which wave
+        // takes which run (the radius only decides the order in which candidates are seen)
+        r2 = r2_first;
     for (int tt = 0; tt < prm.tiles_per_wave; tt++) {
         const int64_t tile = run * prm.tiles_per_wave + tt;
         const int64_t qbase = tile * Q;
@@ -531,8 +566,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             }
 
             // --- pass 1: value histogram over [0, R2] ---------------------------------------
-            for (int b = s; b < NB; b += P) sm.hist[g][b] = 0u;
+            for (int b = s; b < NH * NB; b += P) sm.hist[g][b] = 0u;
             __syncthreads();
+            uint32_t* myhist = &sm.hist[g][(s % NH) * NB];
             bool fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
                 auto t0 = ws.tick();
                 const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, act ? R2 : -1.0f);
@@ -540,7 +576,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 t0 = ws.tick();
                 scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
                     const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
-                    if (d2 <= R2) atomicAdd(&sm.hist[g][vbin(d2, inv_w)], 1u);
+                    if (d2 <= R2) atomicAdd(&myhist[vbin(d2, inv_w)], 1u);
                 });
                 ws.tock(KS_CY_HIST, t0);
                 if (prm.stats) ws.c[KS_CANDIDATES] += (unsigned)(LEAF * wave_sum_int(s == 0 ? len : 0));
@@ -598,16 +634,17 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                     const int nbits = span ? 64 - __clzll((long long)span) : 0;
                     const int shift = nbits > 5 ? nbits - 5 : 0;
                     const float rq = need ? bitsf((uint32_t)(tg[tau].khi >> 32)) : -1.0f;
-                    for (int b = s; b < NB; b += P) sm.hist[g][b] = 0u;
+                    const float rin = need ? bitsf((uint32_t)(tg[tau].klo >> 32)) : -1.0f;
+                    for (int b = s; b < NH * NB; b += P) sm.hist[g][b] = 0u;
                     __syncthreads();
                     fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
-                        const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rq);
+                        const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rq, rin);
                         scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
                             const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
                             if (!(d2 <= R2)) return;
                             const uint64_t key = make_key(d2, __float_as_uint(p.w));
                             if (key >= tg[tau].klo && key <= tg[tau].khi)
-                                atomicAdd(&sm.hist[g][(int)((key - tg[tau].klo) >> shift)], 1u);
+                                atomicAdd(&myhist[(int)((key - tg[tau].klo) >> shift)], 1u);
                         });
                     });
                     if (!fine) {
@@ -722,6 +759,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
         // radius guess for the next tile of this wave
         if (kth_max > 0.0f) r2 = kth_max * prm.r2_margin;
         r2 = fmaxf(r2, 1e-30f);
+    }
     }
     ws.tock(KS_CY_TOTAL, t_wave);
     ws.flush(prm.stats);

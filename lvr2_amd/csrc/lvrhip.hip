@@ -354,14 +354,21 @@ struct Profiler {
     }
 };
 
+// With a work queue the grid is capped at `resident` waves (the ones the device holds at once):
+// they drain the queue; without, one wave per run of tiles.
 template <int P, template <int> class SinkT>
 void launch_tile_p(const BvhView& bv, const QueryView& qv, const TileParams& prm, const SinkT<P>& sink,
-                   hipStream_t st)
+                   hipStream_t st, int64_t resident)
 {
     constexpr int Q = kWave / P;
     const int64_t ntiles = ceil_div(qv.nq, Q);
-    const int64_t grid = ceil_div(ntiles, prm.tiles_per_wave);
+    int64_t grid = ceil_div(ntiles, prm.tiles_per_wave);
     if (grid == 0) return;
+    if (prm.queue) {
+        grid = std::min(grid, resident);
+        fill_u32_kernel<<<1, 64, 0, st>>>(prm.queue, 1, 0u);
+        LVRHIP_LAUNCH_CHECK();
+    }
     knn_tile_kernel<P, SinkT<P>><<<(unsigned)grid, kWave, 0, st>>>(bv, qv, prm, sink);
     LVRHIP_LAUNCH_CHECK();
 }
@@ -385,6 +392,15 @@ struct lvrhip_surface {
     int tiles_per_wave = 8;
     float r2_margin = 1.3f;
     bool merge_retry = false;  // rounds 2..5 in one four-target search
+    // launch shape of the signed-distance searches: lattice vertices are far fewer than points
+    // and vary more in cost per tile, so fewer consecutive tiles per wave balance better
+    int sdf_tiles_per_wave = 2;
+    float sdf_r2_margin = 1.3f;
+    // kNN tile launches: waves take runs of tiles from a work queue (dynamic balance), the grid
+    // capped at what the device holds at once
+    bool use_queue = true;
+    int64_t resident_waves = 4096;
+    DBuf<uint32_t> d_queue;
 
     DBuf<float> px, py, pz;
     DBuf<uint32_t> pidx, rank, qreal;
@@ -518,12 +534,17 @@ struct lvrhip_surface {
     void check_err() { read_err_and_counts(); }
 
     template <template <int> class SinkT>
-    void tile(const QueryView& qv, const TileParams& prm, int P, SinkT<2> s2, SinkT<4> s4, SinkT<8> s8)
+    void tile(const QueryView& qv, const TileParams& prm_in, int P, SinkT<2> s2, SinkT<4> s4, SinkT<8> s8)
     {
+        TileParams prm = prm_in;
+        if (use_queue) {
+            d_queue.reserve(1);
+            prm.queue = d_queue.p;
+        }
         switch (P) {
-            case 2: launch_tile_p<2, SinkT>(view(), qv, prm, s2, st); break;
-            case 4: launch_tile_p<4, SinkT>(view(), qv, prm, s4, st); break;
-            case 8: launch_tile_p<8, SinkT>(view(), qv, prm, s8, st); break;
+            case 2: launch_tile_p<2, SinkT>(view(), qv, prm, s2, st, resident_waves); break;
+            case 4: launch_tile_p<4, SinkT>(view(), qv, prm, s4, st, resident_waves); break;
+            case 8: launch_tile_p<8, SinkT>(view(), qv, prm, s8, st, resident_waves); break;
             default: throw Error(LVRHIP_EINVAL, "lanes per query must be 2, 4 or 8");
         }
     }
@@ -1132,6 +1153,8 @@ struct lvrhip_surface {
         sort_queries(d_q, m);
         prof.stop(tsort, st);
         TileParams prm = params(t0, 0, 2);
+        prm.tiles_per_wave = sdf_tiles_per_wave;
+        prm.r2_margin = sdf_r2_margin;
         prm.stats = kstats_ptr(6);
         QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
         auto mk = [&](auto tag) {
@@ -1325,6 +1348,8 @@ struct lvrhip_surface {
         // 1. SDF reverse: dL/dp (through the means) and dL/dn of every vertex's kd neighbours
         {
             TileParams prm = params(t0, 0, 2);
+            prm.tiles_per_wave = sdf_tiles_per_wave;
+            prm.r2_margin = sdf_r2_margin;
             prm.stats = kstats_ptr(6);
             QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
             auto mk = [&](auto tag) {
@@ -1481,6 +1506,9 @@ int lvrhip_create_scans_device(const float* d_xyz, const int64_t* scan_offsets, 
         h->device = device;
         h->st = (hipStream_t)stream;
         try {
+            int cus = 0;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+                h->resident_waves = (int64_t)cus * 16;
             h->init(d_xyz, scan_offsets, nscans);
             h->build();
             LVRHIP_CHECK(hipStreamSynchronize(h->st));
@@ -1952,6 +1980,21 @@ int lvrhip_autotune(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float
         REQUIRE(h, "NULL handle");
         DeviceGuard dg(h->device);
         h->autotune(lanes_per_query, tiles_per_wave, r2_margin);
+    });
+}
+
+int lvrhip_set_sdf_tuning(lvrhip_t h, int tiles_per_wave, float r2_margin)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        if (tiles_per_wave > 0) {
+            REQUIRE(tiles_per_wave <= 4096, "tiles per wave out of range");
+            h->sdf_tiles_per_wave = tiles_per_wave;
+        }
+        if (r2_margin > 0.0f) {
+            REQUIRE(r2_margin >= 1.0f && r2_margin <= 1e6f, "radius margin out of range");
+            h->sdf_r2_margin = r2_margin;
+        }
     });
 }
 

@@ -181,6 +181,9 @@ class HipSurface:
     def setTuning(self, lanes_per_query: int = 0, tiles_per_wave: int = 0, r2_margin: float = 0.0) -> None:
         check(lib().lvrhip_set_tuning(self._h, int(lanes_per_query), int(tiles_per_wave), float(r2_margin)))
 
+    def setSdfTuning(self, tiles_per_wave: int = 0, r2_margin: float = 0.0) -> None:
+        check(lib().lvrhip_set_sdf_tuning(self._h, int(tiles_per_wave), float(r2_margin)))
+
     def setRetryMode(self, merged: bool) -> None:
         """Adaptive rounds 2..5 in one four-target search (True) or round by round (False)."""
         check(lib().lvrhip_set_retry_mode(self._h, int(bool(merged))))
