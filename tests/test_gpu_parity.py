@@ -372,3 +372,52 @@ def test_large_cloud_32m(oracle_mod):
     sub = np.sort(rng.choice(len(w), 300, replace=False))
     _, _, k_or = o.normals(kn, 3, (0.0, 0.0, 0.0), subset=sub, fp64=True)
     np.testing.assert_array_equal(k_gpu[sub], k_or)
+
+
+@pytest.mark.parametrize("n", [1, 3, 17, 40, 300])
+def test_normals_and_sdf_tiny_clouds(oracle_mod, n):
+    """Clouds smaller than the adaptive k: every round's k is clamped to n (kSearch returns all
+    points), as in the reference; interpolation and SDF on the same clamp."""
+    rng = np.random.default_rng(n)
+    p = (rng.normal(size=(n, 3)) * [3.0, 2.0, 0.3]).astype(np.float32)
+    o = oracle_mod.Surface(p)
+    n32, n64, k_or = o.normals(4, 3, (0.0, 0.0, 10.0), fp64=True)
+    s = _surface(p)
+    s.setKn(4)
+    s.setKi(0)
+    s.setKd(5)
+    s.setFlippoint(0.0, 0.0, 10.0)
+    s.calculateNormals()
+    # the oracle reports the round's nominal k; the library the k actually fitted (at most n)
+    np.testing.assert_array_equal(s.adaptiveK(), np.minimum(k_or, n))
+    g = s.getNormals()
+    if n >= 3:
+        flipdir = np.array([0.0, 0.0, 10.0]) - p.astype(np.float64)
+        amb = np.abs(np.sum(n64 * flipdir, 1)) < 1e-3 * np.linalg.norm(flipdir, axis=1)
+        assert np.quantile(angles(g, n64)[~amb], 0.9) < 1e-3
+    s.setNormals(n32)
+    s.setKi(6)
+    s.interpolateNormals()
+    np.testing.assert_array_equal(s.getNormals(), o.interpolate(n32, 6))
+    s.setNormals(n32)
+    q = (rng.normal(size=(50, 3)) * 2).astype(np.float32)
+    proj, eucl, inv = s.distances(q, 0.5)
+    oproj, oeucl = o.distance(n32, 5, q)
+    np.testing.assert_array_equal(proj, oproj)
+    np.testing.assert_array_equal(eucl, oeucl)
+
+
+def test_empty_cloud_and_empty_queries():
+    from lvr2_amd import LvrHipError
+
+    s = _surface(np.zeros((0, 3), np.float32))
+    s.calculateNormals()
+    assert s.getNormals().shape == (0, 3)
+    with pytest.raises(LvrHipError):
+        s.distances(np.zeros((2, 3), np.float32), 1.0)
+    s2 = _surface(np.random.default_rng(1).normal(size=(100, 3)).astype(np.float32))
+    s2.calculateNormals()
+    proj, eucl, inv = s2.distances(np.zeros((0, 3), np.float32), 1.0)
+    assert proj.shape == (0,)
+    idx, d2 = s2.kSearch(np.zeros((0, 3), np.float32), 4)
+    assert idx.shape == (0, 4)
