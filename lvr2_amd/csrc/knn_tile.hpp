@@ -127,12 +127,14 @@ struct WaveStats {
 
 template <int Q, int BUF, int NT>
 struct TileSmem {
-    float4 cand[CAP];  // x, y, z, original index (as bits)
+    // x, y, z, original index (as bits); slots [CAP, CAP + LEAF) form an all-NaN leaf (chunk
+    // slot CHUNK_LEAVES) that pads the per-query lists to whole steps of the scan
+    float4 cand[CAP + LEAF];
     float4 lbox[CHUNK_LEAVES][2];  // boxes of the chunk's leaves (lo, hi)
     uint32_t hist[Q][NH * NB];
     int stack[STACK_CAP];
-    int tleaf[TL_CAP];                // the tile's leaves (or the current chunk's, when streaming)
-    uint8_t glist[Q][CHUNK_LEAVES];   // per query: chunk leaf slots within its bound
+    int tleaf[TL_CAP];                    // the tile's leaves (or the current chunk's, when streaming)
+    uint8_t glist[Q][CHUNK_LEAVES + 8];   // per query: chunk leaf slots within its bound, then pads
     uint64_t bkey[NT][Q][BUF];
     uint32_t bspos[NT][Q][BUF];
     uint32_t bcnt[NT][Q];
@@ -295,6 +297,8 @@ __device__ __forceinline__ int build_glist(Sm& sm, const BvhView& bv, int base, 
         if (hit) sm.glist[g][len + __popc(gm & ((1u << s) - 1u))] = (uint8_t)j;
         len += __popc(gm);
     }
+    // pads: the NaN leaf, so that scan_glist reads whole steps without bounds tests
+    for (int e = s; e < 4; e += P) sm.glist[g][len + e] = (uint8_t)CHUNK_LEAVES;
     __syncthreads();
     return len;
 }
@@ -351,28 +355,35 @@ __device__ __forceinline__ bool for_each_chunk(Sm& sm, const BvhView& bv, const 
 
 // Scan the points of query group g's listed leaves: fn(p, slot) for each, the P lanes of
 // the group taking every P-th point of a leaf (LEAF / P independent LDS reads per lane).
+// The reads of a step are unconditional (a list's last step runs into its NaN pads), so they
+// are all in flight before the first use; fn must ignore points with NaN coordinates.
 template <int P, class Sm, class Fn>
 __device__ __forceinline__ void scan_glist(const Sm& sm, int g, int s, int len, Fn&& fn)
 {
     constexpr int U = LEAF / P;  // points per lane per leaf
     constexpr int L = U >= 4 ? 1 : 4 / U;  // leaves per step: at least 4 LDS reads in flight
     static_assert(LEAF % P == 0, "lanes per query must divide the leaf size");
+    static_assert(L <= 4, "the lists carry 4 pads");
+    // the next step's list entries are read one step ahead (the rows hold 8 entries past
+    // their last leaf: 4 pads, then up to L - 1 entries that are read but never used)
+    int nxt[L];
+#pragma unroll
+    for (int l = 0; l < L; l++) nxt[l] = sm.glist[g][l];
     for (int j = 0; j < len; j += L) {
         int slot0[L];
         float4 p[L][U];
 #pragma unroll
-        for (int l = 0; l < L; l++) {
-            slot0[l] = (j + l < len) ? (int)sm.glist[g][j + l] * LEAF + s : -1;
-#pragma unroll
-            for (int u = 0; u < U; u++)
-                p[l][u] = slot0[l] >= 0 ? sm.cand[slot0[l] + u * P]
-                                        : make_float4(__builtin_nanf(""), 0.0f, 0.0f, 0.0f);
-        }
+        for (int l = 0; l < L; l++) slot0[l] = nxt[l] * LEAF + s;
 #pragma unroll
         for (int l = 0; l < L; l++)
 #pragma unroll
-            for (int u = 0; u < U; u++)
-                if (slot0[l] >= 0) fn(p[l][u], slot0[l] + u * P);
+            for (int u = 0; u < U; u++) p[l][u] = sm.cand[slot0[l] + u * P];
+#pragma unroll
+        for (int l = 0; l < L; l++) nxt[l] = sm.glist[g][j + L + l];
+#pragma unroll
+        for (int l = 0; l < L; l++)
+#pragma unroll
+            for (int u = 0; u < U; u++) fn(p[l][u], slot0[l] + u * P);
     }
 }
 
@@ -489,6 +500,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
     ws.timing = prm.stats != nullptr;
     const auto t_wave = ws.tick();
     give_lds(sink, sm.cand, 0);
+    if (lane < LEAF) sm.cand[CAP + lane] = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.0f);
     const uint32_t nq = qv.nq_dev ? uniform(*qv.nq_dev) : qv.nq;
     if (prm.area_pp) r2 = fmaxf(uniformf(*prm.area_pp) * (float)prm.t0 / 3.14159265f * prm.r2_margin, 1e-30f);
     const float r2_first = r2;
