@@ -47,6 +47,11 @@ __device__ __forceinline__ int mbcnt(uint64_t m)
 __device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
 __device__ __forceinline__ float bitsf(uint32_t x) { return __uint_as_float(x); }
 
+// min / max of numbers known not to be NaN (a compare and a select; fminf / fmaxf also
+// quiet signalling NaNs, which costs two more instructions each)
+__device__ __forceinline__ float min_num(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float max_num(float a, float b) { return b > a ? b : a; }
+
 // wave-uniform value in an SGPR
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }

@@ -44,6 +44,7 @@ struct BvhView {
     const float* py;
     const float* pz;
     const uint32_t* pidx;  // original index of each sorted point
+    const float4* pts4;    // x, y, z and original index (as bits) of each sorted point
     const float4* leaf_lo;
     const float4* leaf_hi;
     const BvhNode* nodes;
@@ -239,7 +240,7 @@ __device__ __forceinline__ void load_chunk(Sm& sm, const BvhView& bv, int base, 
         v[it] = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.0f);
         if (jl < nl) {
             const uint32_t spos = (uint32_t)sm.tleaf[base + jl] * LEAF + (lane % LEAF);
-            if (spos < bv.n) v[it] = make_float4(bv.px[spos], bv.py[spos], bv.pz[spos], __uint_as_float(bv.pidx[spos]));
+            if (spos < bv.n) v[it] = bv.pts4[spos];
         }
     }
     float4 bl = make_float4(0, 0, 0, 0), bh = make_float4(0, 0, 0, 0);

@@ -181,7 +181,8 @@ __global__ __launch_bounds__(256) void gather_padded_kernel(
     const float* __restrict__ xyz, const int64_t* __restrict__ off, const int64_t* __restrict__ pstart, int ns,
     int64_t npad, const uint32_t* __restrict__ order, const uint64_t* __restrict__ skeys, float* __restrict__ px,
     float* __restrict__ py, float* __restrict__ pz, uint32_t* __restrict__ pidx, uint32_t* __restrict__ rank,
-    uint64_t* __restrict__ pcode, uint32_t* __restrict__ qreal, uint16_t* __restrict__ scan_of)
+    uint64_t* __restrict__ pcode, uint32_t* __restrict__ qreal, uint16_t* __restrict__ scan_of,
+    float4* __restrict__ pts4)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npad) return;
@@ -198,9 +199,11 @@ __global__ __launch_bounds__(256) void gather_padded_kernel(
         rank[j] = (uint32_t)p;
         pcode[p] = skeys[r];
         qreal[r] = (uint32_t)p;
+        pts4[p] = make_float4(xyz[3 * (int64_t)j], xyz[3 * (int64_t)j + 1], xyz[3 * (int64_t)j + 2], __uint_as_float(j));
     } else {
         px[p] = py[p] = pz[p] = __builtin_nanf("");
         pidx[p] = ~0u;
+        pts4[p] = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), __uint_as_float(~0u));
         pcode[p] = cnt > 0 ? skeys[off[sc] + cnt - 1] : 0ull;
     }
     if (scan_of) scan_of[p] = (uint16_t)sc;

@@ -404,6 +404,7 @@ struct lvrhip_surface {
     DBuf<uint32_t> d_queue;
 
     DBuf<float> px, py, pz;
+    DBuf<float4> pts4;  // the same points as (x, y, z, original index bits): one load per point
     DBuf<uint32_t> pidx, rank, qreal;
     DBuf<int64_t> d_off, d_pstart;
     DBuf<uint16_t> scan_of;
@@ -475,6 +476,7 @@ struct lvrhip_surface {
         v.py = py.p;
         v.pz = pz.p;
         v.pidx = pidx.p;
+        v.pts4 = pts4.p;
         v.leaf_lo = leaf_lo.p;
         v.leaf_hi = leaf_hi.p;
         v.nodes = nodes.p;
@@ -587,6 +589,7 @@ struct lvrhip_surface {
         s_v1.reserve(n);
         s_hist.reserve((size_t)RS_RADIX * ceil_div(n, RS_TILE));
         px.reserve(npad);
+        pts4.reserve(npad);
         py.reserve(npad);
         pz.reserve(npad);
         pidx.reserve(npad);
@@ -627,7 +630,8 @@ struct lvrhip_surface {
         radix_sort_pairs(s_k0.p, s_v0.p, s_k1.p, s_v1.p, N, s_hist.p, st);
         const unsigned gp = (unsigned)ceil_div(npad, 256);
         gather_padded_kernel<<<gp, 256, 0, st>>>(src, d_off.p, d_pstart.p, ns, npad, s_v0.p, s_k0.p, px.p, py.p, pz.p,
-                                                 pidx.p, rank.p, s_k1.p, qreal.p, ns > 1 ? scan_of.p : nullptr);
+                                                 pidx.p, rank.p, s_k1.p, qreal.p, ns > 1 ? scan_of.p : nullptr,
+                                                 pts4.p);
         LVRHIP_LAUNCH_CHECK();
         // leaves and hierarchy
         leaf_boxes_kernel<<<gp, 256, 0, st>>>(px.p, py.p, pz.p, npad, s_k1.p, nleaf, leaf_lo.p, leaf_hi.p, s_k0.p);

@@ -169,18 +169,18 @@ struct SinkNormals {
             sx += rx;
             sy += ry;
             sz += rz;
-            sxx += rx * rx;
-            sxy += rx * ry;
-            sxz += rx * rz;
-            syy += ry * ry;
-            syz += ry * rz;
-            szz += rz * rz;
-            mnx = fminf(mnx, x);
-            mny = fminf(mny, y);
-            mnz = fminf(mnz, z);
-            mxx = fmaxf(mxx, x);
-            mxy = fmaxf(mxy, y);
-            mxz = fmaxf(mxz, z);
+            sxx = fmaf(rx, rx, sxx);
+            sxy = fmaf(rx, ry, sxy);
+            sxz = fmaf(rx, rz, sxz);
+            syy = fmaf(ry, ry, syy);
+            syz = fmaf(ry, rz, syz);
+            szz = fmaf(rz, rz, szz);
+            mnx = min_num(mnx, x);
+            mny = min_num(mny, y);
+            mnz = min_num(mnz, z);
+            mxx = max_num(mxx, x);
+            mxy = max_num(mxy, y);
+            mxz = max_num(mxz, z);
         } else {
             const uint32_t slot = atomicAdd(&cnt[g], 1u);
             if (slot < (uint32_t)kiw) ki_list[(size_t)qpos_ * kiw + slot] = spos;
@@ -263,23 +263,23 @@ struct SinkLevels {
     __device__ __forceinline__ void accept(int tau, int, int, float x, float y, float z, uint64_t, uint32_t,
                                            uint32_t*)
     {
-        lo[tau][0] = fminf(lo[tau][0], x);
-        lo[tau][1] = fminf(lo[tau][1], y);
-        lo[tau][2] = fminf(lo[tau][2], z);
-        hi[tau][0] = fmaxf(hi[tau][0], x);
-        hi[tau][1] = fmaxf(hi[tau][1], y);
-        hi[tau][2] = fmaxf(hi[tau][2], z);
+        lo[tau][0] = min_num(lo[tau][0], x);
+        lo[tau][1] = min_num(lo[tau][1], y);
+        lo[tau][2] = min_num(lo[tau][2], z);
+        hi[tau][0] = max_num(hi[tau][0], x);
+        hi[tau][1] = max_num(hi[tau][1], y);
+        hi[tau][2] = max_num(hi[tau][2], z);
         if (tau == 0) {
             const float rx = x - qx_, ry = y - qy_, rz = z - qz_;
             m[0] += rx;
             m[1] += ry;
             m[2] += rz;
-            m[3] += rx * rx;
-            m[4] += rx * ry;
-            m[5] += rx * rz;
-            m[6] += ry * ry;
-            m[7] += ry * rz;
-            m[8] += rz * rz;
+            m[3] = fmaf(rx, rx, m[3]);
+            m[4] = fmaf(rx, ry, m[4]);
+            m[5] = fmaf(rx, rz, m[5]);
+            m[6] = fmaf(ry, ry, m[6]);
+            m[7] = fmaf(ry, rz, m[7]);
+            m[8] = fmaf(rz, rz, m[8]);
         }
     }
 
