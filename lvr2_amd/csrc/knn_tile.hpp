@@ -25,6 +25,8 @@
 // leaves chunk by chunk instead.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "lbvh.hpp"
 
@@ -47,6 +49,7 @@ struct BvhView {
     const float4* pts4;    // x, y, z and original index (as bits) of each sorted point
     const float4* leaf_lo;
     const float4* leaf_hi;
+    const float4* leaf_agg;  // per leaf: its point sums about the box centre (leaf_sums_kernel)
     const BvhNode* nodes;
     int nleaf;
     uint32_t n;
@@ -136,6 +139,7 @@ struct TileSmem {
     int stack[STACK_CAP];
     int tleaf[TL_CAP];                    // the tile's leaves (or the current chunk's, when streaming)
     uint8_t glist[Q][CHUNK_LEAVES + 8];   // per query: chunk leaf slots within its bound, then pads
+    uint8_t flist[Q][CHUNK_LEAVES];       // per query: chunk leaf slots taken whole (collection)
     uint64_t bkey[NT][Q][BUF];
     uint32_t bspos[NT][Q][BUF];
     uint32_t bcnt[NT][Q];
@@ -162,10 +166,11 @@ struct TileBox {
 // continuing the traversal held on the LDS stack (sp entries).  Stops when the stack is
 // empty or the list is full; returns the list length (wave-uniform).  sp = -1 on overflow.
 // With rej != NULL, *rej is lowered to the smallest box distance of a child the traversal
-// rejected (a lower bound of the distance of every point outside the list).
+// rejected (a lower bound of the distance of every point outside the list).  With
+// store = false the leaves are only counted (up to cap).
 template <class Sm>
 __device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const TileBox& tb, float r2, int& sp, int cap,
-                                           WaveStats& ws, float* rej = nullptr)
+                                           WaveStats& ws, float* rej = nullptr, bool store = true)
 {
     const int lane = lane_id();
     int nl = 0;
@@ -211,8 +216,8 @@ __device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const Tile
         const int r0 = mbcnt(bl0), r1 = __popcll(bl0) + mbcnt(bl1);
         bool push0 = (o0 != 0x7fffffff) && (!l0 || r0 >= take);
         bool push1 = (o1 != 0x7fffffff) && (!l1 || r1 >= take);
-        if (l0 && r0 < take) sm.tleaf[nl + r0] = -1 - o0;
-        if (l1 && r1 < take) sm.tleaf[nl + r1] = -1 - o1;
+        if (store && l0 && r0 < take) sm.tleaf[nl + r0] = -1 - o0;
+        if (store && l1 && r1 < take) sm.tleaf[nl + r1] = -1 - o1;
         const uint64_t bp0 = ballot(push0), bp1 = ballot(push1);
         if (push0) sm.stack[sp + mbcnt(bp0)] = o0;
         if (push1) sm.stack[sp + __popcll(bp0) + mbcnt(bp1)] = o1;
@@ -303,6 +308,52 @@ __device__ __forceinline__ int build_glist(Sm& sm, const BvhView& bv, int base, 
     __syncthreads();
     return len;
 }
+
+// build_glist for a collection pass whose sink takes whole leaves: the leaves within sqrt(rq)
+// whose farthest corner is closer than sqrt(rfull) (every point of them has d^2 < rfull) go
+// to sm.flist[g] (nfull of them), the others to sm.glist[g] (the return value).
+template <int P, class Sm>
+__device__ __forceinline__ int build_glist_split(Sm& sm, int nl, int g, int s, float qx, float qy, float qz, float rq,
+                                                 float rfull, int& nfull)
+{
+    constexpr uint64_t GM = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
+    int len = 0;
+    nfull = 0;
+    for (int j0 = 0; j0 < nl; j0 += P) {
+        const int j = j0 + s;
+        bool hit = false, full = false;
+        if (j < nl && rq >= 0.0f) {
+            const float4 lo = sm.lbox[j][0], hi = sm.lbox[j][1];
+            hit = box_d2(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, qx, qy, qz, qx, qy, qz) <= rq;
+            if (hit && rfull > 0.0f) {
+                const float fx = fmaxf(fabsf(qx - lo.x), fabsf(qx - hi.x));
+                const float fy = fmaxf(fabsf(qy - lo.y), fabsf(qy - hi.y));
+                const float fz = fmaxf(fabsf(qz - lo.z), fabsf(qz - hi.z));
+                full = (fx * fx + fy * fy) + fz * fz < rfull;
+            }
+        }
+        const bool part = hit && !full;
+        const uint32_t below = (1u << s) - 1u;
+        const uint32_t pm = (uint32_t)((ballot(part) >> (g * P)) & GM);
+        const uint32_t fm = (uint32_t)((ballot(full) >> (g * P)) & GM);
+        if (part) sm.glist[g][len + __popc(pm & below)] = (uint8_t)j;
+        if (full) sm.flist[g][nfull + __popc(fm & below)] = (uint8_t)j;
+        len += __popc(pm);
+        nfull += __popc(fm);
+    }
+    for (int e = s; e < 4; e += P) sm.glist[g][len + e] = (uint8_t)CHUNK_LEAVES;
+    __syncthreads();
+    return len;
+}
+
+// Sinks that take a whole leaf at once in the collection pass declare kLeafAccept = true and
+// accept_leaf(g, s, count, centre, sums, box).
+template <class S, class = void>
+struct leaf_accepting : std::false_type {
+};
+template <class S>
+struct leaf_accepting<S, std::void_t<decltype(S::kLeafAccept)>> : std::integral_constant<bool, S::kLeafAccept> {
+};
 
 // The candidates of one radius attempt: either the tile's leaf list (ntl >= 0; resident when
 // it fits one chunk) or, after a list overflow, a fresh traversal per pass.
@@ -542,6 +593,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             for (int tau = 0; tau < NT; tau++) sm.scnt[tau][g] = 0u;
         }
         float kth_max = 0.0f;
+        // radius^2 bracket of the tile: r2_floor left a query of the tile short, r2_ceil was
+        // shrunk from (too many leaves); growth stays below r2_ceil, shrinking above r2_floor
+        float r2_floor = 0.0f, r2_ceil = INFINITY;
+        int shrinks = 0;
+        int prev_lead = -1;
 
         for (int attempt = 0; attempt < MAX_ATTEMPTS; attempt++) {
             const uint64_t pending = ballot(active);
@@ -551,6 +607,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             // pending queries within R (per axis) of that lane's query: a tile of scattered
             // queries (a sparse query list) would otherwise span a box far larger than R.
             const int lead = __ffsll((long long)pending) - 1;
+            if (lead != prev_lead) {  // a new cluster: its own bracket
+                r2_floor = 0.0f;
+                r2_ceil = INFINITY;
+                shrinks = 0;
+                prev_lead = lead;
+            }
             const int cur = bv.scan_of ? uniform(shfl(scan, lead)) : 0;
             const float lx = uniformf(shfl(qx, lead)), ly = uniformf(shfl(qy, lead)), lz = uniformf(shfl(qz, lead));
             const float reach = sqrtf(r2);
@@ -578,12 +640,37 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 __syncthreads();
                 const auto t0 = ws.tick();
                 const int nl = fill_leaves(sm, bv, tb, R2, sp, TL_CAP, ws, &rej);
-                ws.tock(KS_CY_TRAV, t0);
                 if (sp < 0) {
                     if (lane == 0) atomicMax(prm.err, 5);
                     return;
                 }
-                cs.ntl = sp == 0 ? nl : -1;
+                const bool overflow = sp > 0;  // the list is full and the traversal unfinished
+                // A radius far larger than the targets need (carried from a sparser tile, the
+                // first guess of a run in a dense region, or a growth step that overshot for a
+                // query off a dense surface) would put tens of thousands of points per query
+                // through the passes: with far more leaves in range than the tile's targets can
+                // use (the queries' t0 points each, their spread over the tile box widening the
+                // union), shrink it by the leaf count (counts grow like R^2 - r2_floor) before
+                // any point is scanned.  At most four times per tile.
+                if (shrinks < 4 && R2 > r2_floor && attempt < MAX_ATTEMPTS / 4) {
+                    const float ext = fmaxf(fmaxf(tb.hx - tb.lx, tb.hy - tb.ly), tb.hz - tb.lz);
+                    const float spread = 1.0f + ext / sqrtf(R2);
+                    const float want = ((float)prm.t0 / (float)LEAF + 1.0f) * spread * spread + (float)Q;
+                    const int big = (int)fminf(8.0f * want, 65536.0f);
+                    int total = nl;
+                    if (overflow) total = fill_leaves(sm, bv, tb, R2, sp, 4 * big, ws, nullptr, false) + nl;
+                    if (total > big) {
+                        // geometric steps: for a query off a surface the count grows faster
+                        r2 = r2_floor + (R2 - r2_floor) * fmaxf(sqrtf(2.0f * want / (float)total), 0.125f);
+                        r2_ceil = R2;
+                        shrinks++;
+                        ws.tock(KS_CY_TRAV, t0);
+                        __syncthreads();
+                        continue;
+                    }
+                }
+                ws.tock(KS_CY_TRAV, t0);
+                cs.ntl = overflow ? -1 : nl;  // the passes then traverse again, chunk by chunk
                 cs.resident = false;
             }
 
@@ -665,7 +752,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                     const float rin = need ? bitsf((uint32_t)(tg[tau].klo >> 32)) : -1.0f;
                     for (int b = s; b < NH * NB; b += P) sm.hist[g][b] = 0u;
                     __syncthreads();
-                    fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
+                    // a streamed pass walks the tree only as far as the widest refined bin reaches
+                    const float rq_all = fminf(uniformf(wave_max(rq)), R2);
+                    fine = for_each_chunk(sm, bv, tb, rq_all, root, cs, ws, [&](int nl, int base) {
                         const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rq, rin);
                         scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
                             const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
@@ -709,11 +798,27 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 for (int tau = 0; tau < NT; tau++) sm.bcnt[tau][g] = 0;
             }
             __syncthreads();
+            // tau 0 takes the leaves that lie wholly below its boundary bin whole, when no other
+            // target needs their points one by one
+            constexpr bool kLeaves = leaf_accepting<Sink>::value;
+            const bool whole = kLeaves && (NT < 2 || prm.t1 <= 0) && (NT < 3 || prm.t2 <= 0) && (NT < 4 || prm.t3 <= 0);
+            const float rfull = (whole && ok) ? tg[0].dlo : -1.0f;
             auto collect = [&](int nl, int base) {
                 auto t0 = ws.tick();
-                const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rc);
+                int nfull = 0;
+                const int len = whole ? build_glist_split<P>(sm, nl, g, s, qx, qy, qz, rc, rfull, nfull)
+                                      : build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rc);
                 ws.tock(KS_CY_GLIST, t0);
                 t0 = ws.tick();
+                if constexpr (kLeaves) {
+                    for (int j = s; j < nfull; j += P) {
+                        const int js = sm.flist[g][j];
+                        const int leaf = sm.tleaf[base + js];
+                        const float4 lo = sm.lbox[js][0], hi = sm.lbox[js][1];
+                        const float4* a = bv.leaf_agg + 3 * (size_t)leaf;
+                        sink.accept_leaf(g, s, a[0], a[1], a[2], lo, hi);
+                    }
+                }
                 scan_glist<P>(sm, g, s, len, [&](const float4& p, int c) {
                     const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
                     if (!(d2 <= R2)) return;
@@ -739,7 +844,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 });
                 ws.tock(KS_CY_COLLECT, t0);
             };
-            fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, collect);
+            fine = for_each_chunk(sm, bv, tb, fminf(uniformf(wave_max(rc)), R2), root, cs, ws, collect);
             if (!fine) {
                 if (lane == 0) atomicMax(prm.err, 5);
                 return;
@@ -800,6 +905,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 if (attempt >= MAX_ATTEMPTS / 2) nr2 = R2 * 4.0f;  // a guard: plain doubling of R
                 if (!(nr2 > R2) || nr2 < 1e-30f) nr2 = fmaxf(R2 * 4.0f, 1e-30f);
                 if (nr2 > 1e37f) nr2 = INFINITY;
+                // below a radius already found too wide, unless shrinking is over
+                if (shrinks < 4 && nr2 > r2_ceil && attempt < MAX_ATTEMPTS / 4)
+                    nr2 = fmaxf(R2 + (r2_ceil - R2) * 0.5f, R2 * 1.01f);
+                r2_floor = R2;
                 r2 = nr2;
             }
             __syncthreads();

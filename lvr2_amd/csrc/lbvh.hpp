@@ -241,6 +241,43 @@ __global__ __launch_bounds__(256) void leaf_boxes_kernel(const float* __restrict
     }
 }
 
+// Point sums of every leaf about its box centre c = (lo + hi) / 2, for consumers that take a
+// whole leaf at once: agg[3 l] = (sum (p - c), count), agg[3 l + 1] = (xx, xy, xz, yy),
+// agg[3 l + 2] = (yz, zz, 0, 0) of sum (p - c)(p - c)^T.  One thread per leaf.
+__device__ __forceinline__ float leaf_centre(float lo, float hi) { return (lo + hi) * 0.5f; }
+
+__global__ __launch_bounds__(256) void leaf_sums_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                        const float* __restrict__ pz, int64_t n,
+                                                        const float4* __restrict__ leaf_lo,
+                                                        const float4* __restrict__ leaf_hi, int nleaf,
+                                                        float4* __restrict__ agg)
+{
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= nleaf) return;
+    const float4 lo = leaf_lo[leaf], hi = leaf_hi[leaf];
+    const float cx = leaf_centre(lo.x, hi.x), cy = leaf_centre(lo.y, hi.y), cz = leaf_centre(lo.z, hi.z);
+    float sx = 0, sy = 0, sz = 0, xx = 0, xy = 0, xz = 0, yy = 0, yz = 0, zz = 0;
+    int cnt = 0;
+    for (int j = 0; j < LEAF; j++) {
+        const int64_t i = (int64_t)leaf * LEAF + j;
+        if (i >= n || !(px[i] == px[i])) continue;  // padding slots
+        const float rx = px[i] - cx, ry = py[i] - cy, rz = pz[i] - cz;
+        sx += rx;
+        sy += ry;
+        sz += rz;
+        xx = fmaf(rx, rx, xx);
+        xy = fmaf(rx, ry, xy);
+        xz = fmaf(rx, rz, xz);
+        yy = fmaf(ry, ry, yy);
+        yz = fmaf(ry, rz, yz);
+        zz = fmaf(rz, rz, zz);
+        cnt++;
+    }
+    agg[3 * (int64_t)leaf] = make_float4(sx, sy, sz, (float)cnt);
+    agg[3 * (int64_t)leaf + 1] = make_float4(xx, xy, xz, yy);
+    agg[3 * (int64_t)leaf + 2] = make_float4(yz, zz, 0.0f, 0.0f);
+}
+
 // Karras 2012: internal node i of the radix tree over nl sorted leaf codes.
 __device__ __forceinline__ int karras_delta(const uint64_t* codes, int nl, int i, int j)
 {

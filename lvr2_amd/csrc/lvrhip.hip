@@ -414,6 +414,7 @@ struct lvrhip_surface {
     DBuf<MortonFrame> frames;
     DBuf<float> area;  // median leaf area per point: the first radius guess
     DBuf<float4> leaf_lo, leaf_hi, nlo, nhi;
+    DBuf<float4> leaf_agg;  // per leaf: its point sums about the box centre (3 float4)
     DBuf<BvhNode> nodes;
     DBuf<int> leaf_parent;
     int nleaf = 0;
@@ -479,6 +480,7 @@ struct lvrhip_surface {
         v.pts4 = pts4.p;
         v.leaf_lo = leaf_lo.p;
         v.leaf_hi = leaf_hi.p;
+        v.leaf_agg = leaf_agg.p;
         v.nodes = nodes.p;
         v.nleaf = nleaf;
         v.n = (uint32_t)npad;
@@ -601,6 +603,7 @@ struct lvrhip_surface {
         }
         leaf_lo.reserve(nleaf);
         leaf_hi.reserve(nleaf);
+        leaf_agg.reserve((size_t)3 * nleaf);
         leaf_parent.reserve(nleaf);
         nodes.reserve(nint);
         nlo.reserve(nint);
@@ -635,6 +638,9 @@ struct lvrhip_surface {
         LVRHIP_LAUNCH_CHECK();
         // leaves and hierarchy
         leaf_boxes_kernel<<<gp, 256, 0, st>>>(px.p, py.p, pz.p, npad, s_k1.p, nleaf, leaf_lo.p, leaf_hi.p, s_k0.p);
+        LVRHIP_LAUNCH_CHECK();
+        leaf_sums_kernel<<<(unsigned)ceil_div(nleaf, 256), 256, 0, st>>>(px.p, py.p, pz.p, npad, leaf_lo.p, leaf_hi.p,
+                                                                         nleaf, leaf_agg.p);
         LVRHIP_LAUNCH_CHECK();
         const int nint = std::max(nleaf - 1, 1);
         fill_u32_kernel<<<(unsigned)ceil_div(nleaf, 256), 256, 0, st>>>((uint32_t*)leaf_parent.p, nleaf, ~0u);

@@ -187,6 +187,35 @@ struct SinkNormals {
         }
     }
 
+    // A leaf all of whose points are in the k-set: its point sums about the box centre c
+    // (leaf_sums_kernel) moved to q: sum (p - q) = S1 + n d and
+    // sum (p - q)(p - q)^T = S2 + d S1^T + S1 d^T + n d d^T with d = c - q; its box is the
+    // exact bounding box of its points.
+    static constexpr bool kLeafAccept = true;
+    __device__ __forceinline__ void accept_leaf(int, int, const float4& a, const float4& b, const float4& c,
+                                                const float4& lo, const float4& hi)
+    {
+        const float dx = leaf_centre(lo.x, hi.x) - qx_, dy = leaf_centre(lo.y, hi.y) - qy_,
+                    dz = leaf_centre(lo.z, hi.z) - qz_;
+        const float n = a.w;
+        const float ndx = n * dx, ndy = n * dy, ndz = n * dz;
+        sx += a.x + ndx;
+        sy += a.y + ndy;
+        sz += a.z + ndz;
+        sxx += b.x + fmaf(2.0f * dx, a.x, ndx * dx);
+        sxy += b.y + fmaf(dx, a.y, fmaf(dy, a.x, ndx * dy));
+        sxz += b.z + fmaf(dx, a.z, fmaf(dz, a.x, ndx * dz));
+        syy += b.w + fmaf(2.0f * dy, a.y, ndy * dy);
+        syz += c.x + fmaf(dy, a.z, fmaf(dz, a.y, ndy * dz));
+        szz += c.y + fmaf(2.0f * dz, a.z, ndz * dz);
+        mnx = min_num(mnx, lo.x);
+        mny = min_num(mny, lo.y);
+        mnz = min_num(mnz, lo.z);
+        mxx = max_num(mxx, hi.x);
+        mxy = max_num(mxy, hi.y);
+        mxz = max_num(mxz, hi.z);
+    }
+
     __device__ __forceinline__ void finish(int g, int s, bool ok, uint32_t qpos, float, float, float, const uint32_t*,
                            const uint32_t* cnt1)
     {
