@@ -75,7 +75,7 @@ struct TileParams {
     const float* area_pp;       // optional device value, area per point: first radius^2 =
                                 // area_pp * t0 / pi * r2_margin (else r2_init)
     float r2_margin;            // next tile's radius^2 = margin * largest k-th d^2 of this tile
-    float grow_min;             // smallest growth of radius^2 after a failed attempt (> 1)
+    float grow_min;             // smallest growth of the disc term R^2 - dmin^2 after a failed attempt
     int* err;                   // set to a nonzero code on capacity overflow
     unsigned long long* stats;  // optional counters, see KS_* below
     uint32_t* queue;            // optional work queue (zeroed before the launch): waves take the
@@ -574,6 +574,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
         // every run starts from the same radius guess: results do not depend on which wave
         // takes which run (the radius only decides the order in which candidates are seen)
         r2 = r2_first;
+        float disc_hint = 0.0f;  // R^2 - dmin^2 that held t0 points for the last resolved queries
     for (int tt = 0; tt < prm.tiles_per_wave; tt++) {
         const int64_t tile = run * prm.tiles_per_wave + tt;
         const int64_t qbase = tile * Q;
@@ -883,25 +884,31 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             // off a surface, R^2 - dmin^2 is what the covered disc grows with); the tile's largest.
             const float est = (ok && kth >= 0.0f) ? dm + (kth - dm) * prm.r2_margin : 0.0f;
             kth_max = fmaxf(kth_max, uniformf(wave_max(est)));
+            const float disc_ok = uniformf(wave_max((ok && kth >= 0.0f) ? kth - dm : 0.0f));
+            if (disc_ok > 0.0f) disc_hint = disc_ok;
             ws.c[KS_RESOLVED] += (unsigned)__popcll(ballot(ok && s == 0));
             active = active && !ok;
             if (ballot(act && !ok) != 0) {  // queries of this scan still short of t: grow R
                 // Each short query estimates the radius^2 that holds t0 points from what it saw:
                 // the count in range grows like R^2 - dmin^2 (a surface seen from a distance
-                // dmin); with nothing in range, the nearest box the traversal left out (else x4).
-                // A blind x4 overshoots badly for a query far from a dense surface (the disc it
-                // then covers holds thousands of points).
+                // dmin), and the smallest growth applies to that disc term; with nothing in
+                // range, the nearest box the traversal left out plus the disc the last resolved
+                // queries needed (else x1.1, else x4).  Growing R^2 itself by a fixed factor
+                // overshoots badly for a query far from a dense surface (the disc it then
+                // covers holds thousands of points).
                 float need = 0.0f;
                 if (act && !ok) {
                     if (in_range > 0) {
                         const float disc = R2 - dm;
-                        need = dm + disc * fmaxf((float)prm.t0 / (float)in_range * 1.25f, 1.5f);
+                        need = dm + disc * fmaxf((float)prm.t0 / (float)in_range * 1.25f, fmaxf(prm.grow_min, 1.5f));
+                    } else if (rej < INFINITY) {
+                        need = disc_hint > 0.0f ? rej + 1.5f * disc_hint : rej * 1.1f;
                     } else {
-                        need = rej < INFINITY ? rej * 1.1f : R2 * 4.0f;
+                        need = R2 * 4.0f;
                     }
                 }
                 float nr2 = uniformf(wave_max(need));
-                nr2 = fminf(fmaxf(nr2, R2 * fmaxf(prm.grow_min, 1.01f)), R2 * 16.0f);
+                nr2 = fminf(fmaxf(nr2, R2 * 1.01f), R2 * 16.0f);
                 if (attempt >= MAX_ATTEMPTS / 2) nr2 = R2 * 4.0f;  // a guard: plain doubling of R
                 if (!(nr2 > R2) || nr2 < 1e-30f) nr2 = fmaxf(R2 * 4.0f, 1e-30f);
                 if (nr2 > 1e37f) nr2 = INFINITY;
