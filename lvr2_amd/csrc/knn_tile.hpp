@@ -54,6 +54,9 @@ struct BvhView {
     int nleaf;
     uint32_t n;
     const int* roots;         // per scan: root node of its subtree (NULL: one scan, root 0)
+    const int* fr_ref;        // per scan: FRONTIER top-level subtrees (NULL: start at the root)
+    const float4* fr_lo;
+    const float4* fr_hi;
     const uint16_t* scan_of;  // per query position: its scan (NULL: one scan)
 };
 
@@ -229,6 +232,43 @@ __device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const Tile
     return nl;
 }
 
+// Starts a traversal of scan cur's subtree for the tile box: the stack receives the top-level
+// subtrees (frontier_kernel) whose box is within sqrt(r2), tested in one wave-wide step, or
+// the subtree's root.  Returns the stack depth (wave-uniform); *rej as in fill_leaves.
+template <class Sm>
+__device__ __forceinline__ int start_traversal(Sm& sm, const BvhView& bv, int cur, const TileBox& tb, float r2,
+                                               float* rej = nullptr)
+{
+    const int lane = lane_id();
+    if (!bv.fr_ref) {
+        const int root = bv.roots ? uniform(bv.roots[cur]) : (bv.nleaf > 1 ? 0 : -1);
+        if (lane == 0) sm.stack[0] = root;
+        __syncthreads();
+        return 1;
+    }
+    const size_t f0 = (size_t)cur * FRONTIER;
+    int sp = 0;
+    float rmin = INFINITY;
+#pragma unroll
+    for (int k = 0; k < FRONTIER / kWave; k++) {
+        const size_t j = f0 + k * kWave + lane;
+        const int ref = bv.fr_ref[j];
+        bool hit = false;
+        if (ref != FR_NONE) {
+            const float4 lo = bv.fr_lo[j], hi = bv.fr_hi[j];
+            const float d = box_d2(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, tb.lx, tb.ly, tb.lz, tb.hx, tb.hy, tb.hz);
+            hit = d <= r2;
+            if (!hit) rmin = fminf(rmin, d);
+        }
+        const uint64_t m = ballot(hit);
+        if (hit) sm.stack[sp + mbcnt(m)] = ref;
+        sp += __popcll(m);
+    }
+    if (rej) *rej = fminf(*rej, uniformf(wave_min(rmin)));
+    __syncthreads();
+    return sp;
+}
+
 // Stage the points of leaves sm.tleaf[base, base + nl) in LDS; slot = j * LEAF + offset.
 // Slots past the end of the cloud get NaN coordinates (never in range).
 template <class Sm>
@@ -366,7 +406,7 @@ struct Cands {
 // The kernel's helpers are all inlined: a call would move the structs it passes by
 // reference (views, counters) into scratch memory.
 template <class Sm, class F>
-__device__ __forceinline__ bool for_each_chunk(Sm& sm, const BvhView& bv, const TileBox& tb, float R2, int root,
+__device__ __forceinline__ bool for_each_chunk(Sm& sm, const BvhView& bv, const TileBox& tb, float R2, int cur,
                                                Cands& cs, WaveStats& ws, F&& f)
 {
     if (cs.ntl >= 0) {
@@ -387,9 +427,7 @@ __device__ __forceinline__ bool for_each_chunk(Sm& sm, const BvhView& bv, const 
         if (cs.ntl <= CHUNK_LEAVES) cs.resident = true;
         return true;
     }
-    int sp = 1;
-    if (lane_id() == 0) sm.stack[0] = root;
-    __syncthreads();
+    int sp = start_traversal(sm, bv, cur, tb, R2);
     for (;;) {
         auto t0 = ws.tick();
         const int nl = fill_leaves(sm, bv, tb, R2, sp, CHUNK_LEAVES, ws);
@@ -619,7 +657,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             const float reach = sqrtf(r2);
             const bool near = fabsf(qx - lx) <= reach && fabsf(qy - ly) <= reach && fabsf(qz - lz) <= reach;
             const bool act = active && scan == cur && near;
-            const int root = bv.roots ? uniform(bv.roots[cur]) : (bv.nleaf > 1 ? 0 : -1);
             // tile box of the pending queries of that scan
             TileBox tb;
             tb.lx = uniformf(wave_min(act ? qx : INFINITY));
@@ -636,10 +673,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             Cands cs;
             float rej = INFINITY;  // nearest box the traversal left out
             {
-                int sp = 1;
-                if (lane == 0) sm.stack[0] = root;
-                __syncthreads();
                 const auto t0 = ws.tick();
+                int sp = start_traversal(sm, bv, cur, tb, R2, &rej);
                 const int nl = fill_leaves(sm, bv, tb, R2, sp, TL_CAP, ws, &rej);
                 if (sp < 0) {
                     if (lane == 0) atomicMax(prm.err, 5);
@@ -680,7 +715,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             __syncthreads();
             uint32_t* myhist = &sm.hist[g][(s % NH) * NB];
             float dmin = INFINITY;  // nearest in-range candidate of this lane
-            bool fine = for_each_chunk(sm, bv, tb, R2, root, cs, ws, [&](int nl, int base) {
+            bool fine = for_each_chunk(sm, bv, tb, R2, cur, cs, ws, [&](int nl, int base) {
                 auto t0 = ws.tick();
                 const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, act ? R2 : -1.0f);
                 ws.tock(KS_CY_GLIST, t0);
@@ -755,7 +790,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                     __syncthreads();
                     // a streamed pass walks the tree only as far as the widest refined bin reaches
                     const float rq_all = fminf(uniformf(wave_max(rq)), R2);
-                    fine = for_each_chunk(sm, bv, tb, rq_all, root, cs, ws, [&](int nl, int base) {
+                    fine = for_each_chunk(sm, bv, tb, rq_all, cur, cs, ws, [&](int nl, int base) {
                         const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rq, rin);
                         scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
                             const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
@@ -845,7 +880,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 });
                 ws.tock(KS_CY_COLLECT, t0);
             };
-            fine = for_each_chunk(sm, bv, tb, fminf(uniformf(wave_max(rc)), R2), root, cs, ws, collect);
+            fine = for_each_chunk(sm, bv, tb, fminf(uniformf(wave_max(rc)), R2), cur, cs, ws, collect);
             if (!fine) {
                 if (lane == 0) atomicMax(prm.err, 5);
                 return;

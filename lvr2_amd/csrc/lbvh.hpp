@@ -375,6 +375,78 @@ __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
+// The top of each scan's subtree cut FRONTIER_LEVELS levels below its root: slot j of level
+// L + 1 holds child (j & 1) of slot j >> 1 of level L (a leaf stays in its even slot, the odd
+// one stays empty), so the FRONTIER slots of the last level cover the subtree's leaves
+// exactly once.  A traversal tests these boxes in one wave-wide step instead of descending
+// the top levels one dependent load at a time.  One block of FRONTIER threads per scan.
+constexpr int FRONTIER_LEVELS = 8;
+constexpr int FRONTIER = 1 << FRONTIER_LEVELS;
+constexpr int FR_NONE = (int)0x80000000;  // an empty slot
+
+__global__ __launch_bounds__(FRONTIER) void frontier_kernel(const BvhNode* __restrict__ nodes,
+                                                            const float4* __restrict__ leaf_lo,
+                                                            const float4* __restrict__ leaf_hi, int nleaf,
+                                                            const int* __restrict__ roots, int* __restrict__ fr_ref,
+                                                            float4* __restrict__ fr_lo, float4* __restrict__ fr_hi)
+{
+    __shared__ int ref[FRONTIER];
+    __shared__ float4 lo[FRONTIER], hi[FRONTIER];
+    const int t = threadIdx.x;
+    const int root = roots ? roots[blockIdx.x] : (nleaf > 1 ? 0 : leaf_ref(0));
+    if (t == 0) {
+        ref[0] = root;
+        if (root < 0 && -1 - root >= nleaf) {
+            ref[0] = FR_NONE;  // an empty scan (its nominal leaf lies past the end)
+        } else if (root < 0) {
+            lo[0] = leaf_lo[-1 - root];
+            hi[0] = leaf_hi[-1 - root];
+        } else {
+            const BvhNode nd = nodes[root];
+            lo[0] = make_float4(fminf(nd.a.x, nd.b.z), fminf(nd.a.y, nd.b.w), fminf(nd.a.z, nd.c.x), 0.0f);
+            hi[0] = make_float4(fmaxf(nd.a.w, nd.c.y), fmaxf(nd.b.x, nd.c.z), fmaxf(nd.b.y, nd.c.w), 0.0f);
+        }
+    } else {
+        ref[t] = FR_NONE;
+    }
+    __syncthreads();
+    for (int level = 0; level < FRONTIER_LEVELS; level++) {
+        const int width = 2 << level;  // slots of the next level
+        int r = FR_NONE;
+        float4 l = make_float4(0, 0, 0, 0), h = make_float4(0, 0, 0, 0);
+        if (t < width) {
+            const int src = ref[t >> 1];
+            if (src >= 0) {
+                const BvhNode nd = nodes[src];
+                if (t & 1) {
+                    r = nd.r.y;
+                    l = make_float4(nd.b.z, nd.b.w, nd.c.x, 0.0f);
+                    h = make_float4(nd.c.y, nd.c.z, nd.c.w, 0.0f);
+                } else {
+                    r = nd.r.x;
+                    l = make_float4(nd.a.x, nd.a.y, nd.a.z, 0.0f);
+                    h = make_float4(nd.a.w, nd.b.x, nd.b.y, 0.0f);
+                }
+            } else if (!(t & 1)) {  // a leaf (or an empty slot) moves down unchanged
+                r = src;
+                l = lo[t >> 1];
+                h = hi[t >> 1];
+            }
+        }
+        __syncthreads();
+        if (t < width) {
+            ref[t] = r;
+            lo[t] = l;
+            hi[t] = h;
+        }
+        __syncthreads();
+    }
+    const size_t o = (size_t)blockIdx.x * FRONTIER + t;
+    fr_ref[o] = ref[t];
+    fr_lo[o] = lo[t];
+    fr_hi[o] = hi[t];
+}
+
 // Root of each scan's subtree: scan s covers leaves [pstart[s] / LEAF, pstart[s+1] / LEAF), a
 // contiguous run sharing the scan's key prefix, which in a radix tree is exactly the leaf
 // range of one node (the lowest ancestor of its first leaf that reaches its last leaf).

@@ -415,6 +415,8 @@ struct lvrhip_surface {
     DBuf<float> area;  // median leaf area per point: the first radius guess
     DBuf<float4> leaf_lo, leaf_hi, nlo, nhi;
     DBuf<float4> leaf_agg;  // per leaf: its point sums about the box centre (3 float4)
+    DBuf<int> fr_ref;       // per scan: the FRONTIER top-level subtrees (frontier_kernel)
+    DBuf<float4> fr_lo, fr_hi;
     DBuf<BvhNode> nodes;
     DBuf<int> leaf_parent;
     int nleaf = 0;
@@ -485,6 +487,9 @@ struct lvrhip_surface {
         v.nleaf = nleaf;
         v.n = (uint32_t)npad;
         v.roots = nscans > 1 ? roots.p : nullptr;
+        v.fr_ref = nleaf > 0 ? fr_ref.p : nullptr;
+        v.fr_lo = fr_lo.p;
+        v.fr_hi = fr_hi.p;
         v.scan_of = nscans > 1 ? scan_of.p : nullptr;
         return v;
     }
@@ -604,6 +609,9 @@ struct lvrhip_surface {
         leaf_lo.reserve(nleaf);
         leaf_hi.reserve(nleaf);
         leaf_agg.reserve((size_t)3 * nleaf);
+        fr_ref.reserve((size_t)FRONTIER * nscans);
+        fr_lo.reserve((size_t)FRONTIER * nscans);
+        fr_hi.reserve((size_t)FRONTIER * nscans);
         leaf_parent.reserve(nleaf);
         nodes.reserve(nint);
         nlo.reserve(nint);
@@ -659,6 +667,9 @@ struct lvrhip_surface {
                                                                           roots.p);
             LVRHIP_LAUNCH_CHECK();
         }
+        frontier_kernel<<<(unsigned)ns, FRONTIER, 0, st>>>(nodes.p, leaf_lo.p, leaf_hi.p, nleaf,
+                                                          ns > 1 ? roots.p : nullptr, fr_ref.p, fr_lo.p, fr_hi.p);
+        LVRHIP_LAUNCH_CHECK();
         // density scale for the first radius guess: median leaf area per point
         const int nsample = std::min(nleaf, MEDIAN_MAX);
         const int stride = std::max(1, nleaf / nsample);
