@@ -40,6 +40,9 @@ constexpr int TL_CAP = 512;    // leaves kept per tile
 constexpr int STACK_CAP = 512;
 constexpr int STACK_WIDE = 256;
 constexpr int MAX_ATTEMPTS = 80;
+constexpr int QUEUE_SEGS = 8;     // work-queue segments: one per XCD
+constexpr int QUEUE_STRIDE = 16;  // their counters 64 bytes apart
+constexpr int QUEUE_CHUNK = 32;   // consecutive runs dealt to one segment at a time
 
 struct BvhView {
     const float* px;
@@ -81,9 +84,9 @@ struct TileParams {
     float grow_min;             // smallest growth of the disc term R^2 - dmin^2 after a failed attempt
     int* err;                   // set to a nonzero code on capacity overflow
     unsigned long long* stats;  // optional counters, see KS_* below
-    uint32_t* queue;            // optional work queue (zeroed before the launch): waves take the
-                                // next run of tiles_per_wave tiles from it; else runs are
-                                // strided over the grid
+    uint32_t* queue;            // optional work queue (QUEUE_SEGS counters QUEUE_STRIDE apart,
+                                // zeroed before the launch): waves take the next run of
+                                // tiles_per_wave tiles from it; else runs are strided over the grid
 };
 
 enum {
@@ -596,15 +599,33 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
     const float r2_first = r2;
     const int64_t ntiles = ((int64_t)nq + Q - 1) / Q;
 
-    // runs of tiles_per_wave consecutive tiles, taken from the work queue (every wave ends once
+    // Runs of tiles_per_wave consecutive tiles, taken from the work queue (every wave ends once
     // it is drained) or strided over the grid (a grid sized for an upper bound of the
-    // device-side count leaves the surplus waves idle)
+    // device-side count leaves the surplus waves idle).  The queue deals blocks of QUEUE_CHUNK
+    // consecutive runs to the QUEUE_SEGS XCDs in turn (workgroups are dealt to the XCDs in
+    // turn too), each XCD with its own counter: neighbouring tiles, which read the same leaves,
+    // share one L2, and the runs are still taken roughly in order (a costly region is not left
+    // to the end).  A wave whose XCD's blocks are drained helps the others in turn.
+    const int64_t nruns = (ntiles + prm.tiles_per_wave - 1) / prm.tiles_per_wave;
+    const int xcd = (int)(blockIdx.x % QUEUE_SEGS);
+    int seg_k = 0;
     int64_t run = -1;
     for (;;) {
         if (prm.queue) {
-            uint32_t r = 0;
-            if (lane == 0) r = atomicAdd(prm.queue, 1u);
-            run = (int64_t)(uint32_t)uniform(shfl((int)r, 0));
+            run = -1;
+            while (seg_k < QUEUE_SEGS) {
+                const int seg = (xcd + seg_k) % QUEUE_SEGS;
+                uint32_t r = 0;
+                if (lane == 0) r = atomicAdd(prm.queue + seg * QUEUE_STRIDE, 1u);
+                r = (uint32_t)uniform(shfl((int)r, 0));
+                const int64_t cand = ((int64_t)(r / QUEUE_CHUNK) * QUEUE_SEGS + seg) * QUEUE_CHUNK + r % QUEUE_CHUNK;
+                if (cand < nruns) {
+                    run = cand;
+                    break;
+                }
+                seg_k++;  // this XCD's runs only grow with r: all taken
+            }
+            if (run < 0) break;
         } else {
             run = run < 0 ? (int64_t)blockIdx.x : run + gridDim.x;
         }

@@ -366,7 +366,7 @@ void launch_tile_p(const BvhView& bv, const QueryView& qv, const TileParams& prm
     if (grid == 0) return;
     if (prm.queue) {
         grid = std::min(grid, resident);
-        fill_u32_kernel<<<1, 64, 0, st>>>(prm.queue, 1, 0u);
+        fill_u32_kernel<<<1, 256, 0, st>>>(prm.queue, QUEUE_SEGS * QUEUE_STRIDE, 0u);
         LVRHIP_LAUNCH_CHECK();
     }
     knn_tile_kernel<P, SinkT<P>><<<(unsigned)grid, kWave, 0, st>>>(bv, qv, prm, sink);
@@ -549,7 +549,7 @@ struct lvrhip_surface {
     {
         TileParams prm = prm_in;
         if (use_queue) {
-            d_queue.reserve(1);
+            d_queue.reserve(QUEUE_SEGS * QUEUE_STRIDE);
             prm.queue = d_queue.p;
         }
         switch (P) {
