@@ -12,7 +12,7 @@
 //   3. gives every query its own list of the chunk's leaves whose box is within that
 //      query's current bound, and scans only those, the P lanes of the query splitting
 //      each leaf's points;
-//   4. per query, counts candidates with d^2 <= R^2 into 32 bins linear in d^2.  If fewer
+//   4. per query, counts candidates with d^2 <= R^2 into NB bins linear in d^2.  If fewer
 //      than t candidates are in range the query stays pending and the tile retries with
 //      2R; otherwise the bin holding the t-th smallest key is found, refined by radix
 //      passes over the 64-bit key (d^2 bits, point index) until it holds at most BUF keys,
@@ -32,7 +32,8 @@
 
 namespace lvrhip {
 
-constexpr int NB = 32;         // histogram bins
+constexpr int NB = 64;         // histogram bins
+constexpr int NB_BITS = 6;     // log2(NB): key bits resolved per refinement pass
 constexpr int NH = 1;          // histograms per query (lane s counts into number s % NH)
 constexpr int CAP = 512;       // candidate slots per chunk
 constexpr int CHUNK_LEAVES = CAP / LEAF;
@@ -497,7 +498,7 @@ __device__ __forceinline__ int find_bin(const uint32_t (*hist)[NH * NB], int g, 
                                         int& cnt)
 {
     constexpr int BPL = NB / P;
-    static_assert(NB % P == 0, "bins per lane");
+    static_assert((1 << NB_BITS) == NB && NB % P == 0, "bins per lane");
     int sum = 0;
 #pragma unroll
     for (int j = 0; j < BPL; j++) sum += hist_bin(hist, g, s * BPL + j);
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
     BvhView bv, QueryView qv, TileParams prm, Sink sink)
 {
     constexpr int Q = kWave / P;
-    constexpr int BUF = 12;
+    constexpr int BUF = 16;
     constexpr int NT = Sink::kTargets;
     static_assert(NT == 2 || NT == 4, "two or four targets");
     __shared__ TileSmem<Q, BUF, NT> sm;
@@ -804,7 +805,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                     ws.c[KS_REFINE] += 1;
                     const uint64_t span = tg[tau].khi - tg[tau].klo;
                     const int nbits = span ? 64 - __clzll((long long)span) : 0;
-                    const int shift = nbits > 5 ? nbits - 5 : 0;
+                    const int shift = nbits > NB_BITS ? nbits - NB_BITS : 0;
                     const float rq = need ? bitsf((uint32_t)(tg[tau].khi >> 32)) : -1.0f;
                     const float rin = need ? bitsf((uint32_t)(tg[tau].klo >> 32)) : -1.0f;
                     for (int b = s; b < NH * NB; b += P) sm.hist[g][b] = 0u;
