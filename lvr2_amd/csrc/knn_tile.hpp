@@ -55,6 +55,7 @@ struct BvhView {
     const float4* leaf_hi;
     const float4* leaf_agg;  // per leaf: its point sums about the box centre (leaf_sums_kernel)
     const BvhNode* nodes;
+    const BvhNode4* nodes4;  // the same nodes with their grandchildren (node4_kernel)
     int nleaf;
     uint32_t n;
     const int* roots;         // per scan: root node of its subtree (NULL: one scan, root 0)
@@ -179,66 +180,75 @@ template <class Sm>
 __device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const TileBox& tb, float r2, int& sp, int cap,
                                            WaveStats& ws, float* rej = nullptr, bool store = true)
 {
+    constexpr int NONE = 0x7fffffff;
     const int lane = lane_id();
     int nl = 0;
     float rmin = INFINITY;
     while (sp > 0) {
         const int room = cap - nl;
         if (room == 0) break;
-        // Wide pops (up to 64 entries, up to 128 pushes) while STACK_WIDE slots are free;
-        // below that, depth-first pops of one entry, whose growth is bounded by the tree
-        // depth (< STACK_WIDE - 2 * kWave).
+        // Wide pops (up to 64 entries, up to 256 pushes) while STACK_WIDE slots are free;
+        // below that, depth-first pops of one entry (up to 4 pushes), whose growth is bounded
+        // by the tree depth.  A popped node's four grandchildren are tested at once (BvhNode4).
         const int free_slots = STACK_CAP - sp;
-        const int w = free_slots >= STACK_WIDE ? min(kWave, sp) : (free_slots >= 2 ? 1 : 0);
+        const int w = free_slots >= STACK_WIDE ? min(kWave, sp) : (free_slots >= 4 ? 1 : 0);
         if (w <= 0) {
             sp = -1;
             return nl;
         }
         ws.c[KS_POPS] += (unsigned)w;
-        int ref = lane < w ? sm.stack[sp - 1 - lane] : 0x7fffffff;
+        const int ref = lane < w ? sm.stack[sp - 1 - lane] : NONE;
         sp -= w;
         __syncthreads();
-        // outputs: up to two per lane
-        int o0 = 0x7fffffff, o1 = 0x7fffffff;
-        if (ref != 0x7fffffff) {
+        int o[4] = {NONE, NONE, NONE, NONE};
+        if (ref != NONE) {
             if (ref < 0) {
-                if (-1 - ref < bv.nleaf) o0 = ref;  // a leaf pushed back: already tested
+                if (-1 - ref < bv.nleaf) o[0] = ref;  // a leaf pushed back: already tested
             } else if (ref < bv.nleaf - 1) {
-                const BvhNode nd = bv.nodes[ref];
-                float d0 = box_d2(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tb.lx, tb.ly, tb.lz, tb.hx, tb.hy,
-                                  tb.hz);
-                float d1 = box_d2(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tb.lx, tb.ly, tb.lz, tb.hx, tb.hy,
-                                  tb.hz);
-                if (d0 <= r2) o0 = nd.r.x;
-                else rmin = fminf(rmin, d0);
-                if (d1 <= r2) o1 = nd.r.y;
-                else rmin = fminf(rmin, d1);
+                const BvhNode4 nd = bv.nodes4[ref];
+                const float* f = reinterpret_cast<const float*>(nd.b);
+                const int rr[4] = {nd.r.x, nd.r.y, nd.r.z, nd.r.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float d = box_d2(f[6 * k], f[6 * k + 1], f[6 * k + 2], f[6 * k + 3], f[6 * k + 4],
+                                           f[6 * k + 5], tb.lx, tb.ly, tb.lz, tb.hx, tb.hy, tb.hz);
+                    if (d <= r2 && rr[k] != EMPTY4) o[k] = rr[k];
+                    else rmin = fminf(rmin, d);
+                }
             }
         }
-        const bool l0 = o0 < 0, l1 = o1 < 0;  // leaf outputs
-        const uint64_t bl0 = ballot(l0), bl1 = ballot(l1);
-        const int nleaves = __popcll(bl0) + __popcll(bl1);
+        // leaf outputs in the order (output slot, lane); the ones past the list's room and the
+        // internal nodes go on the stack
+        uint64_t bl[4];
+        int nleaves = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            bl[k] = ballot(o[k] != NONE && o[k] < 0);
+            nleaves += __popcll(bl[k]);
+        }
         const int take = min(nleaves, room);
-        // leaf order: all first outputs, then all second outputs
-        const int r0 = mbcnt(bl0), r1 = __popcll(bl0) + mbcnt(bl1);
-        bool push0 = (o0 != 0x7fffffff) && (!l0 || r0 >= take);
-        bool push1 = (o1 != 0x7fffffff) && (!l1 || r1 >= take);
-        if (store && l0 && r0 < take) sm.tleaf[nl + r0] = -1 - o0;
-        if (store && l1 && r1 < take) sm.tleaf[nl + r1] = -1 - o1;
-        const uint64_t bp0 = ballot(push0), bp1 = ballot(push1);
-        if (push0) sm.stack[sp + mbcnt(bp0)] = o0;
-        if (push1) sm.stack[sp + __popcll(bp0) + mbcnt(bp1)] = o1;
-        sp += __popcll(bp0) + __popcll(bp1);
+        bool push[4];
+        int before = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool lk = o[k] != NONE && o[k] < 0;
+            const int rk = before + mbcnt(bl[k]);
+            before += __popcll(bl[k]);
+            if (store && lk && rk < take) sm.tleaf[nl + rk] = -1 - o[k];
+            push[k] = o[k] != NONE && (!lk || rk >= take);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint64_t bp = ballot(push[k]);
+            if (push[k]) sm.stack[sp + mbcnt(bp)] = o[k];
+            sp += __popcll(bp);
+        }
         nl += take;
         __syncthreads();
     }
     if (rej) *rej = fminf(*rej, uniformf(wave_min(rmin)));
     return nl;
 }
-
-// Starts a traversal of scan cur's subtree for the tile box: the stack receives the top-level
-// subtrees (frontier_kernel) whose box is within sqrt(r2), tested in one wave-wide step, or
-// the subtree's root.  Returns the stack depth (wave-uniform); *rej as in fill_leaves.
 template <class Sm>
 __device__ __forceinline__ int start_traversal(Sm& sm, const BvhView& bv, int cur, const TileBox& tb, float r2,
                                                float* rej = nullptr)

@@ -25,6 +25,17 @@ struct alignas(16) BvhNode {
 
 __device__ __forceinline__ int leaf_ref(int leaf) { return -1 - leaf; }
 
+// The same hierarchy with every node's grandchildren as its children, for traversals that
+// descend two levels per step: child k (0..3) has the box lo = (f[6k], f[6k+1], f[6k+2]),
+// hi = (f[6k+3], f[6k+4], f[6k+5]) of f = the 24 floats of b.  A child of the binary node
+// that is a leaf takes the first of its two slots; the second is empty (EMPTY4, an inverted
+// box that no distance test passes).
+struct alignas(16) BvhNode4 {
+    float4 b[6];
+    int4 r;
+};
+constexpr int EMPTY4 = (int)0x80000000;
+
 // ---- Morton codes ---------------------------------------------------------------------
 
 __device__ __forceinline__ uint64_t expand21(uint32_t v)
@@ -373,6 +384,47 @@ __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// BvhNode4 of every internal node.  One thread per node.
+__global__ __launch_bounds__(256) void node4_kernel(const BvhNode* __restrict__ nodes, int nint,
+                                                    BvhNode4* __restrict__ node4)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nint) return;
+    const BvhNode nd = nodes[i];
+    float f[24];
+    int r[4];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int c = k ? nd.r.y : nd.r.x;
+        if (c >= 0) {
+            const BvhNode cn = nodes[c];
+            const float g[12] = {cn.a.x, cn.a.y, cn.a.z, cn.a.w, cn.b.x, cn.b.y,
+                                 cn.b.z, cn.b.w, cn.c.x, cn.c.y, cn.c.z, cn.c.w};
+#pragma unroll
+            for (int j = 0; j < 12; j++) f[12 * k + j] = g[j];
+            r[2 * k] = cn.r.x;
+            r[2 * k + 1] = cn.r.y;
+        } else {
+            const float g[6] = {k ? nd.b.z : nd.a.x, k ? nd.b.w : nd.a.y, k ? nd.c.x : nd.a.z,
+                                k ? nd.c.y : nd.a.w, k ? nd.c.z : nd.b.x, k ? nd.c.w : nd.b.y};
+#pragma unroll
+            for (int j = 0; j < 6; j++) f[12 * k + j] = g[j];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                f[12 * k + 6 + j] = INFINITY;
+                f[12 * k + 9 + j] = -INFINITY;
+            }
+            r[2 * k] = c;
+            r[2 * k + 1] = EMPTY4;
+        }
+    }
+    BvhNode4 o;
+#pragma unroll
+    for (int j = 0; j < 6; j++) o.b[j] = make_float4(f[4 * j], f[4 * j + 1], f[4 * j + 2], f[4 * j + 3]);
+    o.r = make_int4(r[0], r[1], r[2], r[3]);
+    node4[i] = o;
 }
 
 // The top of each scan's subtree cut FRONTIER_LEVELS levels below its root: slot j of level

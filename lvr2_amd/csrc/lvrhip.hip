@@ -418,6 +418,7 @@ struct lvrhip_surface {
     DBuf<int> fr_ref;       // per scan: the FRONTIER top-level subtrees (frontier_kernel)
     DBuf<float4> fr_lo, fr_hi;
     DBuf<BvhNode> nodes;
+    DBuf<BvhNode4> nodes4;
     DBuf<int> leaf_parent;
     int nleaf = 0;
 
@@ -484,6 +485,7 @@ struct lvrhip_surface {
         v.leaf_hi = leaf_hi.p;
         v.leaf_agg = leaf_agg.p;
         v.nodes = nodes.p;
+        v.nodes4 = nodes4.p;
         v.nleaf = nleaf;
         v.n = (uint32_t)npad;
         v.roots = nscans > 1 ? roots.p : nullptr;
@@ -614,6 +616,7 @@ struct lvrhip_surface {
         fr_hi.reserve((size_t)FRONTIER * nscans);
         leaf_parent.reserve(nleaf);
         nodes.reserve(nint);
+        nodes4.reserve(nint);
         nlo.reserve(nint);
         nhi.reserve(nint);
         s_flags.reserve(std::max<int64_t>(nint, n));
@@ -665,6 +668,10 @@ struct lvrhip_surface {
         if (ns > 1) {
             scan_roots_kernel<<<(unsigned)ceil_div(ns, 256), 256, 0, st>>>(d_pstart.p, ns, leaf_parent.p, nodes.p,
                                                                           roots.p);
+            LVRHIP_LAUNCH_CHECK();
+        }
+        if (nleaf > 1) {
+            node4_kernel<<<(unsigned)ceil_div(nleaf - 1, 256), 256, 0, st>>>(nodes.p, nleaf - 1, nodes4.p);
             LVRHIP_LAUNCH_CHECK();
         }
         frontier_kernel<<<(unsigned)ns, FRONTIER, 0, st>>>(nodes.p, leaf_lo.p, leaf_hi.p, nleaf,
