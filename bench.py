@@ -123,7 +123,11 @@ def bench_batched(args):
 
     t_eager = timed(eager, args.steps)
     same = bool(torch.equal(out, ref))
-    log(f"[bench] batched eager: {t_eager * 1e3:.1f} ms (bit-equal to per-scan: {same})")
+    # normals of unit length: the largest angle between the batched and the per-scan normals
+    cos = torch.clamp(torch.abs((out.double() * ref.double()).sum(1)), max=1.0)
+    max_angle = float(torch.acos(cos).max())
+    log(f"[bench] batched eager: {t_eager * 1e3:.1f} ms (bit-equal to per-scan: {same}; "
+        f"largest angle to the per-scan normals {max_angle:.2e} rad)")
     tune = b.autotune()
     log(f"[bench] autotuned launch shape: {tune}")
     b.captureGraph(out, rebuild=True)
@@ -140,7 +144,8 @@ def bench_batched(args):
         "per_scan_handles": round(n / t_loop / 1e6, 2), "batched_eager": round(n / t_eager / 1e6, 2),
         "batched_graph": round(n / t_graph / 1e6, 2), "graph_vs_per_scan": round(t_loop / t_graph, 2),
         "ms": {"per_scan": round(t_loop * 1e3, 1), "eager": round(t_eager * 1e3, 1), "graph": round(t_graph * 1e3, 1)},
-        "autotuned": tune, "eager_bit_equal_to_per_scan": same, "data": "synthetic (seeded ray-cast room scans)",
+        "autotuned": tune, "eager_bit_equal_to_per_scan": same, "eager_max_angle_to_per_scan_rad": max_angle,
+        "data": "synthetic (seeded ray-cast room scans)",
     }), flush=True)
 
 
