@@ -353,7 +353,17 @@ __global__ __launch_bounds__(256) void refit_kernel(int nl, const int* __restric
     if (leaf >= nl || nl < 2) return;
     int p = leaf_parent[leaf];
     while (p >= 0) {
+        // The boxes this thread stored at the previous level must have reached memory before
+        // the counter lets the other climber (possibly on another XCD, behind another L2) read
+        // them: wait for every outstanding store, write this XCD's L2 back (the device-scope
+        // fence), wait for that too, then the acquire/release increment.  The compiler's
+        // release sequence for the atomic alone issued the increment with the stores of the
+        // loop's previous iteration and the L2 write-back still in flight.
+        __builtin_amdgcn_s_waitcnt(0);
+        __threadfence();
+        __builtin_amdgcn_s_waitcnt(0);
         uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
         if (old == 0) return;
         int4 r = nodes[p].r;
         float4 l0, h0, l1, h1;
