@@ -250,6 +250,27 @@ def test_room_subset_vs_oracle(oracle_mod, merged):
     assert np.all(np.sum(g.astype(np.float64) * -w.points, 1) >= 0)
 
 
+def test_build_reproducible_room():
+    """Fresh handles on the same cloud give identical adaptive k: the bottom-up refit once raced
+    (a node's box read before the other climber's stores had left its XCD's L2)."""
+    from lvr2_amd import scenes
+
+    w = scenes.room(1600, 625)
+    ref = None
+    for _ in range(4):
+        s = _surface(w.points)
+        s.setKn(w.kn)
+        s.setKi(w.ki)
+        s.setFlippoint(*w.flip)
+        s.calculateNormals()
+        got = (s.normalsStats().copy(), s.adaptiveK().copy())
+        del s
+        if ref is None:
+            ref = got
+        np.testing.assert_array_equal(got[0], ref[0])
+        np.testing.assert_array_equal(got[1], ref[1])
+
+
 def test_room_4m_knn_and_orientation(oracle_mod):
     from lvr2_amd import scenes
 
