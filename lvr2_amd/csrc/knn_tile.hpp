@@ -32,8 +32,8 @@
 
 namespace lvrhip {
 
-constexpr int NB = 64;         // histogram bins
-constexpr int NB_BITS = 6;     // log2(NB): key bits resolved per refinement pass
+constexpr int NB = 128;        // histogram bins
+constexpr int NB_BITS = 7;     // log2(NB): key bits resolved per refinement pass
 constexpr int NH = 1;          // histograms per query (lane s counts into number s % NH)
 constexpr int CAP = 512;       // candidate slots per chunk
 constexpr int CHUNK_LEAVES = CAP / LEAF;
@@ -50,6 +50,7 @@ struct BvhView {
     const float* py;
     const float* pz;
     const uint32_t* pidx;  // original index of each sorted point
+    const uint32_t* rank;  // sorted position of each original index
     const float4* pts4;    // x, y, z and original index (as bits) of each sorted point
     const float4* leaf_lo;
     const float4* leaf_hi;
@@ -148,8 +149,7 @@ struct TileSmem {
     int tleaf[TL_CAP];                    // the tile's leaves (or the current chunk's, when streaming)
     uint8_t glist[Q][CHUNK_LEAVES + 8];   // per query: chunk leaf slots within its bound, then pads
     uint8_t flist[Q][CHUNK_LEAVES];       // per query: chunk leaf slots taken whole (collection)
-    uint64_t bkey[NT][Q][BUF];
-    uint32_t bspos[NT][Q][BUF];
+    uint64_t bkey[NT][Q][BUF];  // boundary keys (their low half, the original index, finds the point)
     uint32_t bcnt[NT][Q];
     uint32_t scnt[NT][Q];  // per-query sink slot counters
 };
@@ -902,10 +902,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                                 sink.accept(tau, g, s, p.x, p.y, p.z, key, slot_spos(sm, base, c), sm.scnt[tau]);
                             } else if (key <= tg[tau].khi) {
                                 const uint32_t slot = atomicAdd(&sm.bcnt[tau][g], 1u);
-                                if (slot < (uint32_t)BUF) {
-                                    sm.bkey[tau][g][slot] = key;
-                                    sm.bspos[tau][g][slot] = slot_spos(sm, base, c);
-                                }
+                                if (slot < (uint32_t)BUF) sm.bkey[tau][g][slot] = key;
                             }
                         }
                     }
@@ -932,7 +929,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                         int rank = 0;
                         for (int i = 0; i < nb; i++) rank += sm.bkey[tau][g][i] < kj ? 1 : 0;
                         if (rank < need) {
-                            const uint32_t sp_ = sm.bspos[tau][g][j];
+                            const uint32_t sp_ = bv.rank[(uint32_t)kj];
                             sink.accept(tau, g, s, bv.px[sp_], bv.py[sp_], bv.pz[sp_], kj, sp_, sm.scnt[tau]);
                         }
                         if (tau == 0 && rank == need - 1) kth = bitsf((uint32_t)(kj >> 32));

@@ -480,6 +480,7 @@ struct lvrhip_surface {
         v.py = py.p;
         v.pz = pz.p;
         v.pidx = pidx.p;
+        v.rank = rank.p;
         v.pts4 = pts4.p;
         v.leaf_lo = leaf_lo.p;
         v.leaf_hi = leaf_hi.p;
