@@ -44,6 +44,7 @@ constexpr int MAX_ATTEMPTS = 80;
 constexpr int QUEUE_SEGS = 8;     // work-queue segments: one per XCD
 constexpr int QUEUE_STRIDE = 16;  // their counters 64 bytes apart
 constexpr int QUEUE_CHUNK = 32;   // consecutive runs dealt to one segment at a time
+constexpr float RUN_R2_MARGIN = 2.0f;  // radius^2 of a run's leaf list over its second tile's first
 
 struct BvhView {
     const float* px;
@@ -281,6 +282,37 @@ __device__ __forceinline__ int start_traversal(Sm& sm, const BvhView& bv, int cu
     if (rej) *rej = fminf(*rej, uniformf(wave_min(rmin)));
     __syncthreads();
     return sp;
+}
+
+// The leaves of a run's list (leaf indices in sm.stack[0, nrun), a superset of every leaf within
+// sqrt(r2) of the tile box) whose box is within sqrt(r2) of the tile box, into sm.tleaf: exactly
+// the leaves a traversal for (tb, r2) finds, without walking the tree.  Returns their count;
+// *rej as in fill_leaves (over the list's rejected leaves).
+template <class Sm>
+__device__ __forceinline__ int filter_leaves(Sm& sm, const BvhView& bv, int nrun, const TileBox& tb, float r2,
+                                             float* rej)
+{
+    const int lane = lane_id();
+    int nl = 0;
+    float rmin = INFINITY;
+    for (int j0 = 0; j0 < nrun; j0 += kWave) {
+        const int j = j0 + lane;
+        bool hit = false;
+        int leaf = 0;
+        if (j < nrun) {
+            leaf = sm.stack[j];
+            const float4 lo = bv.leaf_lo[leaf], hi = bv.leaf_hi[leaf];
+            const float d = box_d2(lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, tb.lx, tb.ly, tb.lz, tb.hx, tb.hy, tb.hz);
+            hit = d <= r2;
+            if (!hit) rmin = fminf(rmin, d);
+        }
+        const uint64_t m = ballot(hit);
+        if (hit) sm.tleaf[nl + mbcnt(m)] = leaf;
+        nl += __popcll(m);
+    }
+    if (rej) *rej = fminf(*rej, uniformf(wave_min(rmin)));
+    __syncthreads();
+    return nl;
 }
 
 // Stage the points of leaves sm.tleaf[base, base + nl) in LDS; slot = j * LEAF + offset.
@@ -645,6 +677,14 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
         // takes which run (the radius only decides the order in which candidates are seen)
         r2 = r2_first;
         float disc_hint = 0.0f;  // R^2 - dmin^2 that held t0 points for the last resolved queries
+        // The run's leaf list (sm.stack[0, nrun)): built once, at its second tile, for the box of
+        // its remaining queries (of one scan) and RUN_R2_MARGIN times that tile's first radius^2;
+        // attempts within it filter the list instead of walking the tree.  A traversal reuses
+        // the stack and ends it.
+        bool run_ok = false;
+        int nrun = 0, run_scan = 0;
+        float run_r2 = 0.0f;
+        TileBox rbox;
     for (int tt = 0; tt < prm.tiles_per_wave; tt++) {
         const int64_t tile = run * prm.tiles_per_wave + tt;
         const int64_t qbase = tile * Q;
@@ -658,6 +698,40 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
         const int scan = (valid && bv.scan_of) ? (int)bv.scan_of[qpos] : 0;
         bool active = valid && prm.t0 > 0;
         ws.c[KS_TILES] += 1;
+        if (tt == 1 && prm.tiles_per_wave > 2) {
+            const auto t0 = ws.tick();
+            const int64_t q0 = qbase, q1 = min((run + 1) * prm.tiles_per_wave * Q, (int64_t)nq);
+            const uint32_t p0 = qv.qsel ? qv.qsel[q0] : (uint32_t)q0;
+            run_scan = bv.scan_of ? uniform((int)bv.scan_of[p0]) : 0;
+            float lx = INFINITY, ly = INFINITY, lz = INFINITY, hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
+            for (int64_t i = q0 + lane; i < q1; i += kWave) {
+                const uint32_t pi = qv.qsel ? qv.qsel[i] : (uint32_t)i;
+                if (bv.scan_of && (int)bv.scan_of[pi] != run_scan) continue;
+                const float x = qv.qx[pi], y = qv.qy[pi], z = qv.qz[pi];
+                lx = fminf(lx, x);
+                ly = fminf(ly, y);
+                lz = fminf(lz, z);
+                hx = fmaxf(hx, x);
+                hy = fmaxf(hy, y);
+                hz = fmaxf(hz, z);
+            }
+            rbox.lx = uniformf(wave_min(lx));
+            rbox.ly = uniformf(wave_min(ly));
+            rbox.lz = uniformf(wave_min(lz));
+            rbox.hx = uniformf(wave_max(hx));
+            rbox.hy = uniformf(wave_max(hy));
+            rbox.hz = uniformf(wave_max(hz));
+            run_r2 = fmaxf(r2, 1e-30f) * RUN_R2_MARGIN;
+            int sp = start_traversal(sm, bv, run_scan, rbox, run_r2);
+            const int nl = fill_leaves(sm, bv, rbox, run_r2, sp, TL_CAP, ws);
+            run_ok = sp == 0;  // a list that fits
+            if (run_ok) {
+                for (int j = lane; j < nl; j += kWave) sm.stack[j] = sm.tleaf[j];
+                nrun = nl;
+            }
+            __syncthreads();
+            ws.tock(KS_CY_TRAV, t0);
+        }
         sink.begin(g, s, qi, qpos, qx, qy, qz);
         if (s == 0) {
 #pragma unroll
@@ -706,8 +780,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
             float rej = INFINITY;  // nearest box the traversal left out
             {
                 const auto t0 = ws.tick();
-                int sp = start_traversal(sm, bv, cur, tb, R2, &rej);
-                const int nl = fill_leaves(sm, bv, tb, R2, sp, TL_CAP, ws, &rej);
+                int sp = 0, nl = 0;
+                if (run_ok && cur == run_scan && R2 <= run_r2 && tb.lx >= rbox.lx && tb.ly >= rbox.ly &&
+                    tb.lz >= rbox.lz && tb.hx <= rbox.hx && tb.hy <= rbox.hy && tb.hz <= rbox.hz) {
+                    nl = filter_leaves(sm, bv, nrun, tb, R2, &rej);
+                } else {
+                    run_ok = false;  // the traversal takes the stack
+                    sp = start_traversal(sm, bv, cur, tb, R2, &rej);
+                    nl = fill_leaves(sm, bv, tb, R2, sp, TL_CAP, ws, &rej);
+                }
                 if (sp < 0) {
                     if (lane == 0) atomicMax(prm.err, 5);
                     return;
