@@ -624,9 +624,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
     BvhView bv, QueryView qv, TileParams prm, Sink sink)
 {
     constexpr int Q = kWave / P;
-    constexpr int BUF = 16;
+    constexpr int BUF = Sink::kTargets == 1 ? 24 : 16;  // boundary keys per query and target
     constexpr int NT = Sink::kTargets;
-    static_assert(NT == 2 || NT == 4, "two or four targets");
+    static_assert(NT == 1 || NT == 2 || NT == 4, "one, two or four targets");
     __shared__ TileSmem<Q, BUF, NT> sm;
     const int lane = lane_id();
     const int g = lane / P, s = lane % P;
@@ -1018,7 +1018,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 }
             }
             __syncthreads();
-            sink.finish(g, s, ok, qpos, qx, qy, qz, sm.scnt[0], sm.scnt[1]);
+            sink.finish(g, s, ok, qpos, qx, qy, qz, sm.scnt[0], sm.scnt[NT > 1 ? 1 : 0]);
             ws.tock(KS_CY_FINISH, t_fin);
             // the query's nearest in-range d^2 (dmin over its lanes)
             float dm = dmin;

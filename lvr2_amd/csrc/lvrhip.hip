@@ -742,18 +742,29 @@ struct lvrhip_surface {
             const uint32_t* qsel = r == 1 ? q0 : lists[(r - 1) & 1];
             const uint32_t* nq_dev = r == 1 ? nullptr : d_cnt.p + r;
             QueryView qv{px.p, py.p, pz.p, qsel, (uint32_t)nq0, nq_dev};
-            auto mk = [&](auto tag) {
-                constexpr int P = decltype(tag)::value;
-                SinkNormals<P> s{};
+            auto fill = [&](auto& s) {
                 s.mom = mom.p;
                 s.retry = r < 5 ? s_flags.p : nullptr;
                 s.ki_list = ki_list.p;
                 s.kiw = prm.t1;
+            };
+            auto mk = [&](auto tag) {
+                SinkNormals<decltype(tag)::value> s{};
+                fill(s);
+                return s;
+            };
+            auto mk1 = [&](auto tag) {  // the rounds without a ki list: one target
+                SinkMoments<decltype(tag)::value> s{};
+                fill(s);
                 return s;
             };
             const int tok = prof.start(r == 1 ? LVRHIP_T_NORMALS_FIRST : LVRHIP_T_NORMALS_RETRY, st);
-            tile<SinkNormals>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
-                              mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+            if (prm.t1 > 0)
+                tile<SinkNormals>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
+                                  mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+            else
+                tile<SinkMoments>(qv, prm, lanes_per_query, mk1(std::integral_constant<int, 2>()),
+                                  mk1(std::integral_constant<int, 4>()), mk1(std::integral_constant<int, 8>()));
             prof.stop(tok, st);
             PlaneFitParams pf{};
             pf.qsel = qsel;

@@ -254,6 +254,14 @@ struct SinkNormals {
     }
 };
 
+// SinkNormals for the adaptive rounds after the first, which carry no ki list: one target,
+// so the tile kernel's per-target state (and its LDS) halves and the boundary buffer holds more
+// keys (the selection of k = 32 kn then rarely needs a refinement pass).
+template <int P>
+struct SinkMoments : SinkNormals<P> {
+    static constexpr int kTargets = 1;
+};
+
 // The later adaptive rounds in one search (AdaptiveKSearchSurface.tcc:187-216, rounds 2..5):
 // targets t0 = 32 kn (round 5), t1 = 16 kn, t2 = 8 kn, t3 = 4 kn (round 2).  Each round's k-set
 // is a prefix of the next one's, so one selection yields every round's bounding box; the
