@@ -190,15 +190,31 @@ __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
         ki_list[(size_t)i * kiw + rank] = s;
     }
     __syncthreads();
-    if (valid && j == 0) {
-        float x = nx[i], y = ny[i], z = nz[i];
-        for (int r = 0; r < kiw; r++) {
-            const uint32_t t = sord[base + r];
-            if (t >= n) continue;
-            x += nx[t];
-            y += ny[t];
-            z += nz[t];
+    // lane j of the group loads the normal of rank j (all loads in flight at once); the sum
+    // then runs in rank order through shuffles, as the reference's loop adds them
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    bool have = false;
+    if (valid && j < kiw) {
+        const uint32_t t = sord[base + j];
+        if (t < n) {
+            vx = nx[t];
+            vy = ny[t];
+            vz = nz[t];
+            have = true;
         }
+    }
+    const int lane0 = (int)(threadIdx.x % kWave) - j;  // the group's first lane in the wave
+    float x = valid ? nx[i] : 0.0f, y = valid ? ny[i] : 0.0f, z = valid ? nz[i] : 0.0f;
+    const uint64_t hv = ballot(have);
+    for (int r = 0; r < kiw; r++) {
+        const float ax = shfl(vx, lane0 + r), ay = shfl(vy, lane0 + r), az = shfl(vz, lane0 + r);
+        if ((hv >> (lane0 + r)) & 1ull) {
+            x += ax;
+            y += ay;
+            z += az;
+        }
+    }
+    if (valid && j == 0) {
         normalize3(x, y, z);
         ox[i] = x;
         oy[i] = y;
