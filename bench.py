@@ -376,7 +376,7 @@ def main():
     achieved = bytes_step / launches_per_step / (avg_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
-        "kernel": f"knn_tile_kernel<{lanes}, SinkNormals<{lanes}>> (all adaptive rounds)",
+        "kernel": f"knn_tile_kernel<{lanes}, SinkNormals<{lanes}> | SinkMoments<{lanes}>> (the five adaptive-round launches)",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

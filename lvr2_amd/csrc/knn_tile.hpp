@@ -40,7 +40,10 @@ constexpr int CHUNK_LEAVES = CAP / LEAF;
 constexpr int TL_CAP = 512;    // leaves kept per tile
 constexpr int STACK_CAP = 512;
 constexpr int STACK_WIDE = 256;
-constexpr int MAX_ATTEMPTS = 80;
+constexpr int MAX_ATTEMPTS = 1024;    // per tile (a safety net: each cluster of queries grows x4 per
+                                      // attempt after CLUSTER_GUARD of its own)
+constexpr int CLUSTER_SHRINK = 12;    // a cluster's first attempts that may shrink R or cap its growth
+constexpr int CLUSTER_GUARD = 24;
 constexpr int QUEUE_SEGS = 8;     // work-queue segments: one per XCD
 constexpr int QUEUE_STRIDE = 16;  // their counters 64 bytes apart
 constexpr int QUEUE_CHUNK = 32;   // consecutive runs dealt to one segment at a time
@@ -356,7 +359,8 @@ __device__ __forceinline__ void load_chunk(Sm& sm, const BvhView& bv, int base, 
 template <class Sm>
 __device__ __forceinline__ uint32_t slot_spos(const Sm& sm, int base, int c)
 {
-    return (uint32_t)sm.tleaf[base + c / LEAF] * LEAF + (uint32_t)(c % LEAF);
+    const uint32_t u = (uint32_t)c;  // slots are non-negative: unsigned division is a shift
+    return (uint32_t)sm.tleaf[base + (int)(u / LEAF)] * LEAF + u % LEAF;
 }
 
 // Per query group g: the chunk's leaves (sm.tleaf[base, base+nl)) whose box lies within
@@ -743,6 +747,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
         float r2_floor = 0.0f, r2_ceil = INFINITY;
         int shrinks = 0;
         int prev_lead = -1;
+        int cl_att = 0;  // attempts of the current cluster
 
         for (int attempt = 0; attempt < MAX_ATTEMPTS; attempt++) {
             const uint64_t pending = ballot(active);
@@ -756,8 +761,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 r2_floor = 0.0f;
                 r2_ceil = INFINITY;
                 shrinks = 0;
+                cl_att = 0;
                 prev_lead = lead;
             }
+            cl_att++;
             const int cur = bv.scan_of ? uniform(shfl(scan, lead)) : 0;
             const float lx = uniformf(shfl(qx, lead)), ly = uniformf(shfl(qy, lead)), lz = uniformf(shfl(qz, lead));
             const float reach = sqrtf(r2);
@@ -801,7 +808,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 // use (the queries' t0 points each, their spread over the tile box widening the
                 // union), shrink it by the leaf count (counts grow like R^2 - r2_floor) before
                 // any point is scanned.  At most four times per tile.
-                if (shrinks < 4 && R2 > r2_floor && attempt < MAX_ATTEMPTS / 4) {
+                if (shrinks < 4 && R2 > r2_floor && cl_att <= CLUSTER_SHRINK) {
                     const float ext = fmaxf(fmaxf(tb.hx - tb.lx, tb.hy - tb.ly), tb.hz - tb.lz);
                     const float spread = 1.0f + ext / sqrtf(R2);
                     const float want = ((float)prm.t0 / (float)LEAF + 1.0f) * spread * spread + (float)Q;
@@ -1054,11 +1061,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 }
                 float nr2 = uniformf(wave_max(need));
                 nr2 = fminf(fmaxf(nr2, R2 * 1.01f), R2 * 16.0f);
-                if (attempt >= MAX_ATTEMPTS / 2) nr2 = R2 * 4.0f;  // a guard: plain doubling of R
+                if (cl_att > CLUSTER_GUARD) nr2 = R2 * 4.0f;  // a guard: plain doubling of R
                 if (!(nr2 > R2) || nr2 < 1e-30f) nr2 = fmaxf(R2 * 4.0f, 1e-30f);
                 if (nr2 > 1e37f) nr2 = INFINITY;
                 // below a radius already found too wide, unless shrinking is over
-                if (shrinks < 4 && nr2 > r2_ceil && attempt < MAX_ATTEMPTS / 4)
+                if (shrinks < 4 && nr2 > r2_ceil && cl_att <= CLUSTER_SHRINK)
                     nr2 = fmaxf(R2 + (r2_ceil - R2) * 0.5f, R2 * 1.01f);
                 r2_floor = R2;
                 r2 = nr2;
