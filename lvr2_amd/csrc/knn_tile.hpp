@@ -450,6 +450,8 @@ struct leaf_accepting<S, std::void_t<decltype(S::kLeafAccept)>> : std::integral_
 struct Cands {
     int ntl;        // leaves in sm.tleaf, or -1 when streaming
     bool resident;  // all candidates already staged in sm.cand
+    int staged;     // first list entry of the chunk staged in LDS (-1: none)
+    bool reverse;   // the next pass walks the chunks backwards (it starts with the staged one)
 };
 
 // Calls f(nl, base) once per staged chunk of nl leaves starting at sm.tleaf[base].
@@ -465,15 +467,22 @@ __device__ __forceinline__ bool for_each_chunk(Sm& sm, const BvhView& bv, const 
             __syncthreads();
             return true;
         }
-        for (int base = 0; base < cs.ntl; base += CHUNK_LEAVES) {
+        // passes alternate direction, so each starts with the chunk the last one left staged
+        const int nch = (cs.ntl + CHUNK_LEAVES - 1) / CHUNK_LEAVES;
+        for (int i = 0; i < nch; i++) {
+            const int base = (cs.reverse ? nch - 1 - i : i) * CHUNK_LEAVES;
             const int nl = min(CHUNK_LEAVES, cs.ntl - base);
-            const auto t0 = ws.tick();
-            load_chunk(sm, bv, base, nl);
-            ws.tock(KS_CY_LOAD, t0);
-            ws.c[KS_CHUNKS] += 1;
+            if (base != cs.staged) {
+                const auto t0 = ws.tick();
+                load_chunk(sm, bv, base, nl);
+                ws.tock(KS_CY_LOAD, t0);
+                ws.c[KS_CHUNKS] += 1;
+                cs.staged = base;
+            }
             f(nl, base);
             __syncthreads();
         }
+        cs.reverse = !cs.reverse;
         if (cs.ntl <= CHUNK_LEAVES) cs.resident = true;
         return true;
     }
@@ -828,6 +837,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 ws.tock(KS_CY_TRAV, t0);
                 cs.ntl = overflow ? -1 : nl;  // the passes then traverse again, chunk by chunk
                 cs.resident = false;
+                cs.staged = -1;
+                cs.reverse = false;
             }
 
             // --- pass 1: value histogram over [0, R2] ---------------------------------------
