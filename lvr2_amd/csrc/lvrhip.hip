@@ -754,6 +754,8 @@ struct lvrhip_surface {
         for (int r = 1; r <= last; r++) {
             const int t0 = clamp_k((int64_t)kn << r);
             TileParams prm = params(t0, r == 1 ? kiw : 0, 0);
+            // the later rounds' query lists are sparse: shorter runs balance their costlier tiles
+            if (r > 1) prm.tiles_per_wave = std::max(1, tiles_per_wave / 2);
             prm.stats = kstats_ptr(r - 1);
             const uint32_t* qsel = r == 1 ? q0 : lists[(r - 1) & 1];
             const uint32_t* nq_dev = r == 1 ? nullptr : d_cnt.p + r;
