@@ -988,7 +988,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                 }
                 scan_glist<P>(sm, g, s, len, [&](const float4& p, int c) {
                     const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
-                    if (!(d2 <= R2)) return;
+                    if (!(d2 <= rc)) return;  // beyond every target's boundary bin (rc <= R2)
 #pragma unroll
                     for (int tau = 0; tau < NT; tau++) {
                         if (tg[tau].t <= 0) continue;
