@@ -229,8 +229,9 @@ int lvrhip_set_profiling(lvrhip_t h, int on);
 int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[128], int reset);
 
 /* Launch shape of the kNN tile kernels (no lvr2 counterpart): lanes per query (2, 4 or 8;
- * queries per wavefront = 64 / lanes), consecutive tiles per wavefront, and the radius
- * margin carried from one tile to the next.  Values <= 0 keep the current setting. */
+ * queries per wavefront = 64 / lanes), consecutive tiles per wavefront (the adaptive rounds
+ * after the first take half as many), and the radius margin carried from one tile to the
+ * next.  Values <= 0 keep the current setting. */
 int lvrhip_set_tuning(lvrhip_t h, int lanes_per_query, int tiles_per_wave, float r2_margin);
 /* ms[i], launches[i] for i < LVRHIP_T_COUNT; either may be NULL.  reset != 0 clears them. */
 int lvrhip_get_timings(lvrhip_t h, double* ms, int64_t* launches, int reset);
