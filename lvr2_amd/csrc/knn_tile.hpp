@@ -925,7 +925,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) v
                         const int len = build_glist<P>(sm, bv, base, nl, g, s, qx, qy, qz, rq, rin);
                         scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
                             const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
-                            if (!(d2 <= R2)) return;
+                            // outside the bin's d^2 range (rq <= R^2; -1 when not refining)
+                            if (!(d2 <= rq && d2 >= rin)) return;
                             const uint64_t key = make_key(d2, __float_as_uint(p.w));
                             if (key >= tg[tau].klo && key <= tg[tau].khi)
                                 atomicAdd(&myhist[(int)((key - tg[tau].klo) >> shift)], 1u);
