@@ -352,6 +352,7 @@ __global__ __launch_bounds__(256) void refit_kernel(int nl, const int* __restric
     int leaf = blockIdx.x * blockDim.x + threadIdx.x;
     if (leaf >= nl || nl < 2) return;
     int p = leaf_parent[leaf];
+    bool stored = false;  // this thread has stored a node's boxes (the leaf level stores none)
     while (p >= 0) {
         // The boxes this thread stored at the previous level must have reached memory before
         // the counter lets the other climber (possibly on another XCD, behind another L2) read
@@ -359,9 +360,11 @@ __global__ __launch_bounds__(256) void refit_kernel(int nl, const int* __restric
         // fence), wait for that too, then the acquire/release increment.  The compiler's
         // release sequence for the atomic alone issued the increment with the stores of the
         // loop's previous iteration and the L2 write-back still in flight.
-        __builtin_amdgcn_s_waitcnt(0);
-        __threadfence();
-        __builtin_amdgcn_s_waitcnt(0);
+        if (stored) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __threadfence();
+            __builtin_amdgcn_s_waitcnt(0);
+        }
         uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         if (old == 0) return;
@@ -386,6 +389,7 @@ __global__ __launch_bounds__(256) void refit_kernel(int nl, const int* __restric
         nodes[p].c = make_float4(l1.z, h1.x, h1.y, h1.z);
         node_lo[p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.0f);
         node_hi[p] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.0f);
+        stored = true;
         p = r.z;
     }
 }
