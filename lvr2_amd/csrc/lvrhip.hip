@@ -613,7 +613,7 @@ struct lvrhip_surface {
         s_k1.reserve(std::max(n, npad));
         s_v0.reserve(n);
         s_v1.reserve(n);
-        s_hist.reserve((size_t)RS_RADIX * ceil_div(n, RS_TILE));
+        s_hist.reserve(rs_hist_size(n));
         px.reserve(npad);
         pts4.reserve(npad);
         py.reserve(npad);
@@ -1181,7 +1181,7 @@ struct lvrhip_surface {
         s_k1.reserve(std::max<int64_t>(m, 1));
         s_v0.reserve(std::max<int64_t>(m, 1));
         s_v1.reserve(std::max<int64_t>(m, 1));
-        s_hist.reserve((size_t)RS_RADIX * ceil_div(std::max<int64_t>(m, 1), RS_TILE));
+        s_hist.reserve(rs_hist_size(m));
         const unsigned gb = (unsigned)ceil_div(m, 256);
         morton_queries_kernel<<<gb, 256, 0, st>>>(d_q, m, frames.p, s_k0.p, s_v0.p);
         LVRHIP_LAUNCH_CHECK();
@@ -1256,7 +1256,7 @@ struct lvrhip_surface {
 
     void lat_sort(uint64_t* k, uint32_t* v, int64_t m, int end_bit = 64)
     {
-        g_hist.reserve((size_t)RS_RADIX * ceil_div(std::max<int64_t>(m, 1), RS_TILE));
+        g_hist.reserve(rs_hist_size(m));
         radix_sort_pairs(k, v, g_k1.p, g_v1.p, m, g_hist.p, st, end_bit);
     }
 

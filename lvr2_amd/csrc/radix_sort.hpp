@@ -7,6 +7,8 @@
 // its 4096 keys with wave-level match masks (8 ballots per key) and per-wave digit counts.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace lvrhip {
@@ -33,52 +35,113 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint64_t* __r
     ghist[(int64_t)tid * ntiles + blockIdx.x] = h[tid];
 }
 
-// In-place exclusive scan of a[0..len) with one 1024-thread block, 4096 elements per step.
+// One 4096-entry segment of an in-place exclusive scan by a 1024-thread block: entries
+// [base, base + 4096) get carry + their exclusive prefix within the segment; returns the
+// segment's total (on every thread).
+constexpr int SCAN_SEG = 4096;
+__device__ __forceinline__ uint32_t scan_segment(uint32_t* __restrict__ a, int64_t len, int64_t base, uint32_t carry,
+                                                 uint32_t* wsum)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t v[4];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int64_t i = base + (int64_t)tid * 4 + j;
+        v[j] = i < len ? a[i] : 0u;
+        s += v[j];
+    }
+    // inclusive wave scan of per-thread sums
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+    for (int w = 0; w < 16; w++) {
+        uint32_t x = wsum[w];
+        if (w < wave) wpre += x;
+        tot += x;
+    }
+    uint32_t run = carry + wpre + incl - s;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int64_t i = base + (int64_t)tid * 4 + j;
+        if (i < len) a[i] = run;
+        run += v[j];
+    }
+    __syncthreads();  // wsum is reused by the next segment
+    return tot;
+}
+
+// In-place exclusive scan of a[0..len) with one 1024-thread block, one segment per step.
 __global__ __launch_bounds__(1024) void scan_exclusive_kernel(uint32_t* __restrict__ a, int64_t len,
                                                               uint32_t* __restrict__ total)
 {
     __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry_s;
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < len; base += SCAN_SEG) carry += scan_segment(a, len, base, carry, wsum);
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+// Multi-block form for long tables (the radix sort's digit table: 256 entries per 4096 keys):
+// (1) each block sums its segment, (2) one block scans the segment sums, (3) each block scans
+// its segment from its carry.
+__global__ __launch_bounds__(1024) void scan_segsum_kernel(const uint32_t* __restrict__ a, int64_t len,
+                                                           uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) carry_s = 0;
-    __syncthreads();
-    for (int64_t base = 0; base < len; base += 4096) {
-        uint32_t v[4];
-        uint32_t s = 0;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_SEG;
+    uint32_t s = 0;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            int64_t i = base + (int64_t)tid * 4 + j;
-            v[j] = i < len ? a[i] : 0u;
-            s += v[j];
-        }
-        // inclusive wave scan of per-thread sums
-        uint32_t incl = s;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        uint32_t wpre = 0, tot = 0;
-        for (int w = 0; w < 16; w++) {
-            uint32_t x = wsum[w];
-            if (w < wave) wpre += x;
-            tot += x;
-        }
-        const uint32_t carry = carry_s;
-        uint32_t run = carry + wpre + incl - s;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            int64_t i = base + (int64_t)tid * 4 + j;
-            if (i < len) a[i] = run;
-            run += v[j];
-        }
-        __syncthreads();
-        if (tid == 0) carry_s = carry + tot;
-        __syncthreads();
+    for (int j = 0; j < 4; j++) {
+        int64_t i = base + (int64_t)tid * 4 + j;
+        s += i < len ? a[i] : 0u;
     }
-    if (tid == 0 && total) *total = carry_s;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) wsum[wave] = s;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < 16; w++) t += wsum[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(1024) void scan_segapply_kernel(uint32_t* __restrict__ a, int64_t len,
+                                                             const uint32_t* __restrict__ carry)
+{
+    __shared__ uint32_t wsum[16];
+    scan_segment(a, len, (int64_t)blockIdx.x * SCAN_SEG, carry[blockIdx.x], wsum);
+}
+
+// Entries the radix sort's table must hold for n keys: the digit table, then the segment sums.
+inline size_t rs_hist_size(int64_t n)
+{
+    const int64_t len = (int64_t)RS_RADIX * ceil_div(std::max<int64_t>(n, 1), RS_TILE);
+    return (size_t)(len + ceil_div(len, SCAN_SEG));
+}
+
+// Exclusive scan of a[0..len) in place; sums holds ceil(len / 4096) entries.
+inline void scan_exclusive(uint32_t* a, int64_t len, uint32_t* sums, hipStream_t st)
+{
+    const int64_t nseg = ceil_div(len, SCAN_SEG);
+    if (nseg <= 4) {
+        scan_exclusive_kernel<<<1, 1024, 0, st>>>(a, len, nullptr);
+        LVRHIP_LAUNCH_CHECK();
+        return;
+    }
+    scan_segsum_kernel<<<(unsigned)nseg, 1024, 0, st>>>(a, len, sums);
+    LVRHIP_LAUNCH_CHECK();
+    scan_exclusive_kernel<<<1, 1024, 0, st>>>(sums, nseg, nullptr);
+    LVRHIP_LAUNCH_CHECK();
+    scan_segapply_kernel<<<(unsigned)nseg, 1024, 0, st>>>(a, len, sums);
+    LVRHIP_LAUNCH_CHECK();
 }
 
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t* __restrict__ kin,
@@ -131,7 +194,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t* 
 }
 
 // Sort n pairs by key bits [0, 64).  tmp buffers must hold n entries; hist must hold
-// 256 * ceil(n / RS_TILE) entries.  The result ends in (keys, vals).
+// rs_hist_size(n) entries.  The result ends in (keys, vals).
 inline void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, int64_t n,
                              uint32_t* hist, hipStream_t st, int end_bit = 64)
 {
@@ -145,8 +208,7 @@ inline void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uin
     for (int shift = 0; shift < end_bit; shift += 8, passes++) {
         rs_hist_kernel<<<ntiles, RS_THREADS, 0, st>>>(kin, n, shift, hist, ntiles);
         LVRHIP_LAUNCH_CHECK();
-        scan_exclusive_kernel<<<1, 1024, 0, st>>>(hist, (int64_t)RS_RADIX * ntiles, nullptr);
-        LVRHIP_LAUNCH_CHECK();
+        scan_exclusive(hist, (int64_t)RS_RADIX * ntiles, hist + (size_t)RS_RADIX * ntiles, st);
         rs_scatter_kernel<<<ntiles, RS_THREADS, 0, st>>>(kin, vin, kout, vout, n, shift, hist, ntiles);
         LVRHIP_LAUNCH_CHECK();
         std::swap(kin, kout);
