@@ -166,20 +166,24 @@ constexpr int INTERP_THREADS = 256;
 template <int G>
 __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
     const float* __restrict__ nx, const float* __restrict__ ny, const float* __restrict__ nz,
-    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    const uint32_t* __restrict__ pidx, uint32_t* __restrict__ ki_list, int kiw, int64_t n, float* __restrict__ ox,
+    const float4* __restrict__ pts4, uint32_t* __restrict__ ki_list, int kiw, int64_t n, float* __restrict__ ox,
     float* __restrict__ oy, float* __restrict__ oz)
 {
     __shared__ uint64_t skey[INTERP_THREADS];
     __shared__ uint32_t sord[INTERP_THREADS];
     const int grp = threadIdx.x / G, j = threadIdx.x % G, base = grp * G;
     const int64_t i = (int64_t)blockIdx.x * (INTERP_THREADS / G) + grp;
-    const bool valid = i < n && pidx[i] != ~0u;  // padding slots have no normal
+    // positions with their point index in w (one 16-byte load per neighbour)
+    const float4 q = i < n ? pts4[i] : make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(~0u));
+    const bool valid = i < n && __float_as_uint(q.w) != ~0u;  // padding slots have no normal
     uint32_t s = 0;
     uint64_t key = ~0ull;
     if (valid && j < kiw) {
         s = ki_list[(size_t)i * kiw + j];
-        if (s < n) key = make_key(d2_contract(px[i], py[i], pz[i], px[s], py[s], pz[s]), pidx[s]);
+        if (s < n) {
+            const float4 p = pts4[s];
+            key = make_key(d2_contract(q.x, q.y, q.z, p.x, p.y, p.z), __float_as_uint(p.w));
+        }
         skey[base + j] = key;
     }
     __syncthreads();
@@ -951,13 +955,13 @@ struct lvrhip_surface {
         const int tok = prof.start(LVRHIP_T_INTERPOLATE, st);
         if (kiw <= 16) {
             interpolate_kernel<16><<<(unsigned)ceil_div(R, INTERP_THREADS / 16), INTERP_THREADS, 0, st>>>(
-                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, R, ox, oy, oz);
+                nx.p, ny.p, nz.p, pts4.p, ki_list.p, kiw, R, ox, oy, oz);
         } else if (kiw <= 32) {
             interpolate_kernel<32><<<(unsigned)ceil_div(R, INTERP_THREADS / 32), INTERP_THREADS, 0, st>>>(
-                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, R, ox, oy, oz);
+                nx.p, ny.p, nz.p, pts4.p, ki_list.p, kiw, R, ox, oy, oz);
         } else {
             interpolate_kernel<64><<<(unsigned)ceil_div(R, INTERP_THREADS / 64), INTERP_THREADS, 0, st>>>(
-                nx.p, ny.p, nz.p, px.p, py.p, pz.p, pidx.p, ki_list.p, kiw, R, ox, oy, oz);
+                nx.p, ny.p, nz.p, pts4.p, ki_list.p, kiw, R, ox, oy, oz);
         }
         LVRHIP_LAUNCH_CHECK();
         prof.stop(tok, st);
