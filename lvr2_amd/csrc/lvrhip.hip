@@ -171,6 +171,7 @@ __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
 {
     __shared__ uint64_t skey[INTERP_THREADS];
     __shared__ uint32_t sord[INTERP_THREADS];
+    __shared__ float4 snrm[INTERP_THREADS];
     const int grp = threadIdx.x / G, j = threadIdx.x % G, base = grp * G;
     const int64_t i = (int64_t)blockIdx.x * (INTERP_THREADS / G) + grp;
     // positions with their point index in w (one 16-byte load per neighbour)
@@ -178,14 +179,20 @@ __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
     const bool valid = i < n && __float_as_uint(q.w) != ~0u;  // padding slots have no normal
     uint32_t s = 0;
     uint64_t key = ~0ull;
+    bool real = false;
     if (valid && j < kiw) {
         s = ki_list[(size_t)i * kiw + j];
         if (s < n) {
             const float4 p = pts4[s];
             key = make_key(d2_contract(q.x, q.y, q.z, p.x, p.y, p.z), __float_as_uint(p.w));
+            real = true;
         }
         skey[base + j] = key;
     }
+    // entries that name a point rank first (their keys are distinct and below ~0)
+    const int lane0 = (int)(threadIdx.x % kWave) - j;  // the group's first lane in the wave
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const int nv = __popcll((ballot(real) >> lane0) & gmask);
     __syncthreads();
     if (valid && j < kiw) {
         int rank = 0;
@@ -194,31 +201,38 @@ __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
         ki_list[(size_t)i * kiw + rank] = s;
     }
     __syncthreads();
-    // lane j of the group loads the normal of rank j (all loads in flight at once); the sum
-    // then runs in rank order through shuffles, as the reference's loop adds them
-    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-    bool have = false;
-    if (valid && j < kiw) {
+    // lane j of the group loads the normal of rank j (all loads in flight at once) into LDS
+    if (j < nv) {
         const uint32_t t = sord[base + j];
-        if (t < n) {
-            vx = nx[t];
-            vy = ny[t];
-            vz = nz[t];
-            have = true;
-        }
+        snrm[threadIdx.x] = make_float4(nx[t], ny[t], nz[t], 0.0f);
     }
-    const int lane0 = (int)(threadIdx.x % kWave) - j;  // the group's first lane in the wave
-    float x = valid ? nx[i] : 0.0f, y = valid ? ny[i] : 0.0f, z = valid ? nz[i] : 0.0f;
-    const uint64_t hv = ballot(have);
-    for (int r = 0; r < kiw; r++) {
-        const float ax = shfl(vx, lane0 + r), ay = shfl(vy, lane0 + r), az = shfl(vz, lane0 + r);
-        if ((hv >> (lane0 + r)) & 1ull) {
-            x += ax;
-            y += ay;
-            z += az;
-        }
-    }
+    __syncthreads();
+    // the group's first lane adds them in rank order, as the reference's loop does
     if (valid && j == 0) {
+        float x = nx[i], y = ny[i], z = nz[i];
+        const float4* sn = &snrm[base];
+        int r = 0;
+        for (; r + 4 <= nv; r += 4) {
+            const float4 a = sn[r], b = sn[r + 1], c = sn[r + 2], d = sn[r + 3];
+            x += a.x;
+            y += a.y;
+            z += a.z;
+            x += b.x;
+            y += b.y;
+            z += b.z;
+            x += c.x;
+            y += c.y;
+            z += c.z;
+            x += d.x;
+            y += d.y;
+            z += d.z;
+        }
+        for (; r < nv; r++) {
+            const float4 a = sn[r];
+            x += a.x;
+            y += a.y;
+            z += a.z;
+        }
         normalize3(x, y, z);
         ox[i] = x;
         oy[i] = y;
