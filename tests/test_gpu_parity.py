@@ -53,6 +53,22 @@ def test_knn_small_clouds(oracle_mod, n, k):
     np.testing.assert_array_equal(gd, od)
 
 
+@pytest.mark.parametrize("n", [262144, 262145, 1000003])
+def test_knn_across_sort_scan_sizes(oracle_mod, n):
+    # The radix sort's digit table holds 256 entries per 4096 keys and is scanned by one block up
+    # to 16384 entries (n <= 262144), else in 4096-entry segments: 262145 points leave a
+    # 256-entry last segment, 1000003 points 15 full segments and a ragged one.
+    rng = np.random.default_rng(n)
+    p = rng.uniform(-5.0, 5.0, size=(n, 3)).astype(np.float32)
+    p[: n // 4] = np.round(p[: n // 4] * 4) / 4  # exact ties and duplicates
+    q = np.concatenate([p[rng.integers(0, n, 200)], 10 * rng.normal(size=(30, 3)).astype(np.float32)])
+    s = _surface(p)
+    gi, gd = s.kSearch(q, 8)
+    oi, od = oracle_mod.knn_brute(p, q, 8)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+
+
 def test_knn_golden_fixtures():
     import os
 
