@@ -166,7 +166,7 @@ constexpr int INTERP_THREADS = 256;
 template <int G>
 __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
     const float* __restrict__ nx, const float* __restrict__ ny, const float* __restrict__ nz,
-    const float4* __restrict__ pts4, uint32_t* __restrict__ ki_list, int kiw, int64_t n, float* __restrict__ ox,
+    const float4* __restrict__ pts4, const uint32_t* __restrict__ ki_list, int kiw, int64_t n, float* __restrict__ ox,
     float* __restrict__ oy, float* __restrict__ oz)
 {
     __shared__ uint64_t skey[INTERP_THREADS];
@@ -198,7 +198,6 @@ __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
         int rank = 0;
         for (int t = 0; t < kiw; t++) rank += skey[base + t] < key ? 1 : 0;
         sord[base + rank] = s;
-        ki_list[(size_t)i * kiw + rank] = s;
     }
     __syncthreads();
     // lane j of the group loads the normal of rank j (all loads in flight at once) into LDS
