@@ -70,16 +70,17 @@ def cpu_baseline(w, gpu_normals, verts, sample_pts: int, sample_q: int):
 
 
 def bench_batched(args):
-    """The batched form on the scan project: 64 scans of one room from random scanner positions,
-    sizes log-uniform in [50,000, 2,000,000], per-scan flip points, kn 50 ki 20.  Compares one
-    handle per scan (the reference's per-scan loop, ScanProjectUtils.cpp:361-410) with one
-    batched handle, eager and replayed from a hipGraph (LBVH build included in every variant)."""
+    """The batched form on the scan project: 64 scans of the 200 x 200 m outdoor scene, each from
+    its own scanner pose, sizes log-uniform in [50,000, 2,000,000] (about 30M points), per-scan
+    flip points, kn 50 ki 20.  Compares one handle per scan (the reference's per-scan loop,
+    ScanProjectUtils.cpp:361-410) with one batched handle, eager and replayed from a hipGraph
+    (LBVH build included in every variant)."""
     import torch
 
     from lvr2_amd import HipSurface, scenes
 
     log("[bench] generating the scan project")
-    sp = scenes.scan_project(args.scans)
+    sp = scenes.scan_project_outdoor(args.scans)
     n = len(sp.points)
     dev = torch.device("cuda:0")
     pts = torch.from_numpy(sp.points).to(dev)
@@ -145,7 +146,7 @@ def bench_batched(args):
         "batched_graph": round(n / t_graph / 1e6, 2), "graph_vs_per_scan": round(t_loop / t_graph, 2),
         "ms": {"per_scan": round(t_loop * 1e3, 1), "eager": round(t_eager * 1e3, 1), "graph": round(t_graph * 1e3, 1)},
         "autotuned": tune, "eager_bit_equal_to_per_scan": same, "eager_max_angle_to_per_scan_rad": max_angle,
-        "data": "synthetic (seeded ray-cast room scans)",
+        "data": "synthetic (seeded ray-cast scans of the outdoor scene)",
     }), flush=True)
 
 
@@ -210,17 +211,17 @@ def bench_adjoint(args):
 
 
 def bench_large(args):
-    """The large-cloud configuration: 32,000,000 points (the room scan on an 8000 x 4000 ray
-    grid, 3 mm noise), kn 100, ki 30, kd 10, voxel 0.1: LBVH build, normals + interpolation,
-    lattice and SDF, each timed."""
+    """The large single cloud: 32,000,000 points ray-cast from the centre of the 200 x 200 x 30 m
+    outdoor scene (Perlin-noise terrain, 500 box buildings), 3 mm noise, kn 100, ki 30, kd 10,
+    voxel 0.1: LBVH build, normals + interpolation, lattice and SDF, each timed."""
     import torch
 
     from lvr2_amd import HipSurface, scenes
 
-    pts = scenes.room_scan(2, 8000, 4000, 40, 0.003)
-    n = len(pts)
-    t_pts = torch.from_numpy(pts).cuda()
-    kn, ki, kd, voxel = 100, 30, 10, 0.1
+    log("[bench] generating the outdoor scan")
+    w = scenes.large_outdoor(args.points or 32_000_000)
+    n = len(w.points)
+    t_pts = torch.from_numpy(w.points).cuda()
 
     def sync_time(fn):
         torch.cuda.synchronize()
@@ -231,26 +232,26 @@ def bench_large(args):
 
     HipSurface(t_pts).freeGPU()  # warm-up
     t_build, s = sync_time(lambda: HipSurface(t_pts))
-    s.setKn(kn)
-    s.setKi(ki)
-    s.setKd(kd)
-    s.setFlippoint(0, 0, 0)
+    s.setKn(w.kn)
+    s.setKi(w.ki)
+    s.setKd(w.kd)
+    s.setFlippoint(*w.flip)
     s.calculateNormals()
     s.interpolateNormals()
     t_norm, _ = sync_time(lambda: (s.calculateNormals(), s.interpolateNormals()))
-    t_lat, (v, _, _) = sync_time(lambda: s.grid(voxel, True, device=True))
-    s.distances(v, voxel)
-    t_sdf, _ = sync_time(lambda: s.distances(v, voxel))
+    t_lat, (v, _, _) = sync_time(lambda: s.grid(w.voxel, True, device=True))
+    s.distances(v, w.voxel)
+    t_sdf, _ = sync_time(lambda: s.distances(v, w.voxel))
     rounds = [int(x) for x in s.normalsStats()]
     log(f"[bench] large: build {t_build:.1f} ms, normals+interp {t_norm:.1f} ms, lattice {t_lat:.1f} ms, "
         f"sdf {t_sdf:.1f} ms for {v.shape[0]} vertices")
     print(json.dumps({
-        "metric": "large cloud normals + interpolation Mpts/s (32M points, kn 100)", "form": "large",
+        "metric": f"large cloud normals + interpolation Mpts/s ({n} points, kn {w.kn})", "form": "large",
         "unit": "Mpts/s", "n_gpus": 1, "points": n, "value": round(n / t_norm * 1e3 / 1e6, 2),
         "sdf_mqps": round(v.shape[0] / t_sdf * 1e3 / 1e6, 2), "vertices": int(v.shape[0]),
         "ms": {"build": round(t_build, 1), "normals_interp": round(t_norm, 1), "lattice": round(t_lat, 1),
                "sdf": round(t_sdf, 1)}, "adaptive_rounds": rounds,
-        "data": "synthetic (seeded ray-cast room scan, 8000 x 4000 rays)",
+        "data": "synthetic (seeded ray-cast outdoor scene: Perlin terrain + 500 buildings, 200 x 200 x 30 m)",
     }), flush=True)
 
 
@@ -271,6 +272,7 @@ def main():
                     help="plain: the room-scan headline line; batched / adjoint: the two other forms; "
                          "large: the 32M-point cloud")
     ap.add_argument("--scans", type=int, default=64)
+    ap.add_argument("--points", type=int, default=0, help="points of the large form (default 32,000,000)")
     args = ap.parse_args()
     if args.form != "plain" and args.gpus == 1:
         {"batched": bench_batched, "adjoint": bench_adjoint, "large": bench_large}[args.form](args)
@@ -354,6 +356,17 @@ def main():
     t_sdf = sum(e[2].elapsed_time(e[3]) for e in evs) / args.steps
     ms_step = sum(e[0].elapsed_time(e[3]) for e in evs) / args.steps
     tm = s.timings()
+    kss = s.kernelStats()
+    ks, kw = kss["sdf"], kss["merged_rounds"]
+    if ks["tiles"]:
+        log(f"[bench] SDF search: {ks['resolved'] / args.steps:.0f} vertices one lane each, "
+            f"{ks['refine'] / args.steps:.0f} left to the wave pass; per vertex {ks['candidates'] / max(ks['resolved'], 1):.1f} "
+            f"points scanned, {ks['pops'] / max(ks['resolved'], 1):.1f} nodes expanded; slowest lane per wave "
+            f"{16 * ks['attempts'] / ks['tiles']:.1f} points; {ks['cy_total'] / ks['tiles']:.0f} cycles per wave")
+    if kw["tiles"]:
+        log(f"[bench] SDF wave pass: {kw['resolved'] / args.steps:.0f} vertices, {kw['refine'] / args.steps:.0f} left "
+            f"to the tile kernel; per vertex {kw['candidates'] / kw['tiles']:.0f} points, {kw['pops'] / kw['tiles']:.0f} "
+            f"nodes, {kw['cy_total'] / kw['tiles']:.0f} cycles")
     rounds = s.normalsStats()
     mpts = n / ((t_norm + t_int) / 1e3) / 1e6
     sdf_mqps = m / (t_sdf / 1e3) / 1e6

@@ -5,10 +5,17 @@
 // thrown as std::runtime_error whose message contains "out of memory" for device allocation
 // failures, the substring LargeScaleReconstruction matches (LargeScaleReconstruction.tcc:656-662).
 //
+// calculateNormals() returns what CudaSurface::calculateNormals returns to lvr2's callers
+// (lvr2_reconstruct/Main.cpp:369-370, LargeScaleReconstruction.tcc:654-673): the estimated
+// normals interpolated with ki inside the same call, as CudaSurface::GPU_NN does
+// (CudaSurface.cu:1135-1159); interpolateNormals() is empty there (CudaSurface.cu:1255-1257)
+// and here.  The normals are those of AdaptiveKSearchSurface::calculateSurfaceNormals
+// (AdaptiveKSearchSurface.tcc:158-290), not CudaSurface's approximate kd-tree window.
+//
 // The point and query types are template parameters so that the header compiles without the
 // lvr2 headers: a points array is anything with get() (boost::shared_array<float>) or with
-// width / elements (LBPointArray<float>), a query point anything with m_position.{x,y,z},
-// m_distance and m_invalid (QueryPoint<BaseVector<float>>).
+// width / dim / elements (LBPointArray<float>), a query point anything with
+// m_position.{x,y,z}, m_distance and m_invalid (QueryPoint<BaseVector<float>>).
 #ifndef LVR2_AMD_HIPSURFACE_HPP
 #define LVR2_AMD_HIPSURFACE_HPP
 
@@ -59,18 +66,41 @@ public:
 
     ~HipSurface() { freeGPU(); }
 
-    void calculateNormals() { check(lvrhip_calculate_normals(m_h)); }
+    // CudaSurface::calculateNormals (CudaSurface.cu:1238-1253): adaptive-k normals with the
+    // flip, then the interpolation with ki when ki > 0 (GPU_NN, CudaSurface.cu:1143-1148).
+    void calculateNormals()
+    {
+        check(lvrhip_calculate_normals(m_h));
+        if (m_ki > 0) check(lvrhip_interpolate_normals(m_h));
+    }
 
-    void interpolateNormals() { check(lvrhip_interpolate_normals(m_h)); }
+    // CudaSurface::interpolateNormals is empty (CudaSurface.cu:1255-1257): calculateNormals
+    // has interpolated already.
+    void interpolateNormals() {}
 
-    // getNormals(floatArr output_normals): n * 3 floats in input order
-    template <class FloatArr>
+    // getNormals(floatArr output_normals) (CudaSurface.cu:1097-1104): n * 3 floats in input order
+    template <class FloatArr, class = decltype(std::declval<FloatArr&>().get())>
     void getNormals(FloatArr output_normals)
     {
         check(lvrhip_get_normals(m_h, output_normals.get()));
     }
 
+    // getNormals(LBPointArray<float>&) (CudaSurface.cu:1083-1095): sets width and dim and
+    // fills the caller's elements (n * 3 floats, allocated by the caller)
+    template <class PointArray, class = decltype(std::declval<PointArray&>().elements)>
+    void getNormals(PointArray& output_normals)
+    {
+        output_normals.dim = 3;
+        output_normals.width = (decltype(output_normals.width))m_n;
+        check(lvrhip_get_normals(m_h, output_normals.elements));
+    }
+
     void getNormals(float* output_normals) { check(lvrhip_get_normals(m_h, output_normals)); }
+
+    // Given normals (n * 3 floats, input order) instead of calculateNormals, for callers whose
+    // point buffer already has normals and that only need distances
+    // (LargeScaleReconstruction.tcc:640, 680-687).  No CudaSurface counterpart.
+    void setNormals(const float* normals) { check(lvrhip_set_normals(m_h, normals)); }
 
     void setKn(int kn)
     {
@@ -131,6 +161,18 @@ public:
     void kSearch(const float* queries, size_t m, int k, int32_t* idx, float* d2)
     {
         check(lvrhip_knn(m_h, queries, m, k, idx, d2));
+    }
+
+    // SearchTree::radiusSearch for many queries: the k nearest with d^2 < r^2 (count: m).
+    void radiusSearch(const float* queries, size_t m, int k, float r, int32_t* idx, float* d2, int32_t* count)
+    {
+        check(lvrhip_radius_search(m_h, queries, m, k, r, idx, d2, count));
+    }
+
+    // PointsetSurface::getInterpolatedNormal at m positions (out: m * 3 floats).
+    void getInterpolatedNormals(const float* positions, size_t m, float* out)
+    {
+        check(lvrhip_interpolated_normals(m_h, positions, m, out));
     }
 
     void freeGPU()

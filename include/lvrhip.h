@@ -150,6 +150,30 @@ int lvrhip_get_normals_device(lvrhip_t h, float* d_out);
  * NULL. */
 int lvrhip_knn(lvrhip_t h, const float* q, size_t m, int k, int32_t* out_idx, float* out_d2);
 
+/* Replaces SearchTree::radiusSearch (SearchTree.hpp:79-92; SearchKDTree::radiusSearch,
+ * KDTree.tcc:368-386, which searches the k nearest with d^2 < r^2, KDTree.tcc:101-110, 227-237):
+ * per query the k smallest keys among the points with fp32 d^2 < fp32 (r * r), ascending in
+ * out_idx / out_d2 (m*k entries, padded with -1 / +inf), out_count[i] = how many were found.
+ * out_d2 and out_count may be NULL.  Distances are squared (the reference returns their
+ * square roots). */
+int lvrhip_radius_search(lvrhip_t h, const float* q, size_t m, int k, float r, int32_t* out_idx, float* out_d2,
+                         int32_t* out_count);
+
+/* Replaces PointsetSurface::getInterpolatedNormal (PointsetSurface.tcc:61-74) for m positions:
+ * (0, 1, 0) (Normal's default, Normal.hpp:62) plus the normalised normals of the ki nearest
+ * points in ascending key order, divided by ki, not normalised again.  out_xyz: m*3 floats.
+ * Requires normals and ki >= 1 (lvrhip_set_k). */
+int lvrhip_interpolated_normals(lvrhip_t h, const float* q, size_t m, float* out_xyz);
+
+/* Search algorithm of the searches with at most 32 neighbours (distances, kSearch, radius
+ * search, interpolated normals at positions); no lvr2 counterpart.  LVRHIP_SEARCH_AUTO (the
+ * default) runs them best-first, one lane per query; the queries whose traversal stack
+ * overflows are searched again by one wave each, and any that outgrow that by the tile kernel.
+ * LVRHIP_SEARCH_TILE forces the tile kernel, LVRHIP_SEARCH_WAVE the wave-per-query search for
+ * every query.  All give identical results. */
+enum { LVRHIP_SEARCH_AUTO = 0, LVRHIP_SEARCH_TILE = 1, LVRHIP_SEARCH_WAVE = 2 };
+int lvrhip_set_search_mode(lvrhip_t h, int mode);
+
 /* Replaces CudaSurface::distances (CudaSurface.cu:1273-1317) with the semantics of
  * AdaptiveKSearchSurface::distance (tcc:414-457) and the invalid rule of
  * PointsetGrid::calcDistanceValues (PointsetGrid.tcc:138-143): for each of the m
@@ -212,16 +236,18 @@ enum {
     LVRHIP_T_NORMALS_FIRST = 0, /* kNN tile kernel of the normals, first adaptive round */
     LVRHIP_T_NORMALS_RETRY = 1, /* the same kernel in the later adaptive rounds */
     LVRHIP_T_INTERPOLATE = 2,   /* interpolation kernel */
-    LVRHIP_T_SDF_KERNEL = 3,    /* fused kNN + signed distance kernel */
+    LVRHIP_T_SDF_KERNEL = 3,    /* fused kNN + signed distance kernel (best-first, or the tile kernel) */
     LVRHIP_T_SDF_SORT = 4,      /* Morton sort of the query vertices */
     LVRHIP_T_BUILD = 5,         /* LBVH build */
-    LVRHIP_T_COUNT = 6
+    LVRHIP_T_SDF_REDO = 6,      /* tile kernel for the vertices whose best-first search overflowed */
+    LVRHIP_T_COUNT = 7
 };
 int lvrhip_set_profiling(lvrhip_t h, int on);
 
 /* Kernel work counters collected while profiling is on (no lvr2 counterpart):
  * out[g * 16 + c] for group g (0..4 the normals' adaptive rounds 1..5, 5 the merged later
- * rounds, 6 SDF, 7 other searches: kSearch, the ki list, the adjoint) and counter c (0 tiles,
+ * rounds and the wave-per-query pass of the signed distances, 6 SDF, 7 other searches:
+ * kSearch, the ki list, the adjoint) and counter c (0 tiles,
  * 1 radius attempts, 2 LDS-resident attempts, 3 candidate points scanned in the histogram
  * pass, 4 refinement passes, 5 chunk loads, 6 queries resolved, 7 traversal node pops;
  * 8..15 shader-clock cycles summed over waves in: traversal, chunk staging, per-query leaf

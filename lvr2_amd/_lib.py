@@ -55,13 +55,17 @@ _SIGS = {
     "lvrhip_grid_get_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lvrhip_get_retry_mode": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "lvrhip_distances_backward_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
+    "lvrhip_radius_search": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_float, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]),
+    "lvrhip_interpolated_normals": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "lvrhip_set_search_mode": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 STAT_GROUPS = ("round1", "round2", "round3", "round4", "round5", "merged_rounds", "sdf", "other")
 STAT_COUNTERS = ("tiles", "attempts", "resident", "candidates", "refine", "chunks", "resolved", "pops",
                  "cy_trav", "cy_load", "cy_glist", "cy_hist", "cy_select", "cy_collect", "cy_finish", "cy_total")
 
-TIMERS = ("normals_first", "normals_retry", "interpolate", "sdf_kernel", "sdf_sort", "build")
+TIMERS = ("normals_first", "normals_retry", "interpolate", "sdf_kernel", "sdf_sort", "build", "sdf_redo")
 
 EXPORTED = tuple(_SIGS)
 

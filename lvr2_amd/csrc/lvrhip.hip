@@ -6,6 +6,7 @@
 #include "lbvh.hpp"
 #include "knn_tile.hpp"
 #include "sinks.hpp"
+#include "knn_bestfirst.hpp"
 #include "adjoint.hpp"
 #include "lattice.hpp"
 
@@ -13,7 +14,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +26,28 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// Every device allocation of the library.  With LVRHIP_FAIL_ALLOC_AFTER=n in the environment
+// it refuses, on the host, every allocation after the first n made since the variable took its
+// current value, returning hipErrorOutOfMemory as a full device does: callers can exercise the
+// out-of-memory contract (the error text LargeScaleReconstruction.tcc:656-662 matches) without
+// filling a device.
+hipError_t dev_malloc(void** p, size_t bytes)
+{
+    if (const char* v = std::getenv("LVRHIP_FAIL_ALLOC_AFTER")) {
+        static std::mutex mu;
+        static std::string seen;
+        static long long made = 0;
+        std::lock_guard<std::mutex> lk(mu);
+        if (seen != v) {
+            seen = v;
+            made = 0;
+        }
+        if (made >= std::atoll(v)) return hipErrorOutOfMemory;
+        made++;
+    }
+    return hipMalloc(p, bytes);
+}
+
 template <typename T>
 struct DBuf {
     T* p = nullptr;
@@ -31,7 +56,7 @@ struct DBuf {
     {
         if (n <= cap) return;
         release();
-        if (n) LVRHIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+        if (n) LVRHIP_CHECK(dev_malloc(reinterpret_cast<void**>(&p), n * sizeof(T)));
         cap = n;
     }
     void release()
@@ -59,6 +84,13 @@ struct DeviceGuard {
         if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
     }
 };
+
+inline uint32_t host_fbits(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return b;
+}
 
 // ---- small kernels of the host pipeline ---------------------------------------------------------
 
@@ -310,6 +342,102 @@ __global__ __launch_bounds__(kWave) void sort_rows_kernel(const uint64_t* __rest
     }
 }
 
+// radiusSearch from sorted kSearch rows: the neighbours with d^2 < r^2 are a prefix of each row;
+// the rest become padding (-1 / +inf) and count[r] = the prefix length.
+__global__ void radius_filter_kernel(int64_t m, int k, float r2, int32_t* __restrict__ idx, float* __restrict__ d2,
+                                     int32_t* __restrict__ count)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    int found = 0;
+    for (int j = 0; j < k; j++) {
+        const size_t o = (size_t)r * k + j;
+        const bool in = idx[o] >= 0 && d2[o] < r2;
+        if (in) {
+            found++;
+        } else {
+            idx[o] = -1;
+            d2[o] = INFINITY;
+        }
+    }
+    if (count) count[r] = found;
+}
+
+// getInterpolatedNormal (PointsetSurface.tcc:61-74) from sorted kSearch rows of t <= ki input
+// indices; normals in sorted order, found through the inverse permutation.
+__global__ void mean_normal_rows_kernel(int64_t m, int k, int t, const int32_t* __restrict__ idx,
+                                        const uint32_t* __restrict__ rank, const float* __restrict__ nx,
+                                        const float* __restrict__ ny, const float* __restrict__ nz, int ki,
+                                        float* __restrict__ out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    float rx = 0.0f, ry = 1.0f, rz = 0.0f;
+    for (int j = 0; j < t; j++) {
+        const int32_t id = idx[(size_t)r * k + j];
+        if (id < 0) continue;  // padding (a cloud with NaN points)
+        const uint32_t s = rank[id];
+        float x = nx[s], y = ny[s], z = nz[s];
+        if (!(x == 0.0f && y == 0.0f && z == 0.0f)) {
+            const float l = sqrtf(x * x + y * y + z * z);
+            x /= l;
+            y /= l;
+            z /= l;
+        }
+        rx += x;
+        ry += y;
+        rz += z;
+    }
+    const float c = (float)ki;
+    out[3 * r] = rx / c;
+    out[3 * r + 1] = ry / c;
+    out[3 * r + 2] = rz / c;
+}
+
+// Best-first kNN launch for t <= BF_KMAX: the list length is the smallest of 8, 10, 16, 32 that
+// holds t (10: the signed distance's usual kd).
+template <class Out>
+void launch_bf(const BvhView& bv, const float* qx, const float* qy, const float* qz, int64_t m, int t, uint64_t empty,
+               const Out& out, hipStream_t st, unsigned long long* stats, uint32_t* queue, int64_t resident,
+               uint2* spill)
+{
+    if (m == 0) return;
+    int64_t grid = ceil_div(m, BF_THREADS);
+    if (queue) {
+        grid = std::min(grid, resident);
+        fill_u32_kernel<<<1, 256, 0, st>>>(queue, QUEUE_SEGS * QUEUE_STRIDE, 0u);
+        LVRHIP_LAUNCH_CHECK();
+    }
+    const unsigned g = (unsigned)grid;
+    if (t <= 8)
+        bf_knn_kernel<8, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, m, t, empty, out, stats, queue, spill);
+    else if (t <= 10)
+        bf_knn_kernel<10, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, m, t, empty, out, stats, queue, spill);
+    else if (t <= 16)
+        bf_knn_kernel<16, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, m, t, empty, out, stats, queue, spill);
+    else
+        bf_knn_kernel<32, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, m, t, empty, out, stats, queue, spill);
+    LVRHIP_LAUNCH_CHECK();
+}
+
+// The wave-per-query pass over list[0, *count) (the queries the lane pass left).
+template <class Out>
+void launch_bf_wave(const BvhView& bv, const float* qx, const float* qy, const float* qz, const uint32_t* list,
+                    const uint32_t* count, int t, uint64_t empty, const Out& out, hipStream_t st,
+                    unsigned long long* stats)
+{
+    const unsigned grid = BF_WAVE_BLOCKS;
+    if (t <= 8)
+        bf_knn_wave_kernel<8, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, list, count, t, empty, out, stats);
+    else if (t <= 10)
+        bf_knn_wave_kernel<10, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, list, count, t, empty, out, stats);
+    else if (t <= 16)
+        bf_knn_wave_kernel<16, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, list, count, t, empty, out, stats);
+    else
+        bf_knn_wave_kernel<32, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, list, count, t, empty, out, stats);
+    LVRHIP_LAUNCH_CHECK();
+}
+
 // ki list without normals (used when ki > 2*kn, so the first normal round cannot supply it);
 // unordered, like the one written by SinkNormals.
 template <int P>
@@ -435,6 +563,12 @@ struct lvrhip_surface {
     bool use_queue = true;
     int64_t resident_waves = 4096;
     DBuf<uint32_t> d_queue;
+    // searches with at most BF_KMAX neighbours (distances, kSearch, radius search, interpolated
+    // normals at positions) run best-first unless the tile kernel is forced
+    int search_mode = LVRHIP_SEARCH_AUTO;
+    int64_t bf_resident = 4096;  // best-first waves the device holds at once
+    DBuf<uint32_t> s_redo, s_redo_list;
+    DBuf<uint2> s_spill;  // the lane search's stack entries beyond its LDS part
 
     DBuf<float> px, py, pz;
     DBuf<float4> pts4;  // the same points as (x, y, z, original index bits): one load per point
@@ -467,7 +601,16 @@ struct lvrhip_surface {
     DBuf<uint8_t> kround;
     DBuf<float> mom;  // per position: moments of its k-set about the point (9 floats)
     DBuf<uint32_t> s_bsum;
-    DBuf<uint32_t> d_cnt;  // [r] = queries of adaptive round r (r = 2..5), counted on the device
+    // device counters, allocated and zeroed with the handle: [r] = queries of adaptive round r
+    // (r = 2..5), [6] temporary list lengths of the merged rounds and the adjoint, [7] the
+    // lattice's, [8] queries a best-first search left to the tile kernel
+    static constexpr int CNT_SLOTS = 16;
+    DBuf<uint32_t> d_cnt;
+    uint32_t* counter(int i) const
+    {
+        if (!d_cnt.p || i < 0 || i >= CNT_SLOTS) throw Error(LVRHIP_ESTATE, "internal: device counter unavailable");
+        return d_cnt.p + i;
+    }
     int64_t nq_first = 0;  // queries of the first round of the last normals pass
     int64_t n_round[5] = {0, 0, 0, 0, 0};
     DBuf<int> err;
@@ -615,6 +758,9 @@ struct lvrhip_surface {
         err.reserve(1);
         fill_u32_kernel<<<1, 64, 0, st>>>((uint32_t*)err.p, 1, 0u);
         LVRHIP_LAUNCH_CHECK();
+        d_cnt.reserve(CNT_SLOTS);
+        fill_u32_kernel<<<1, 64, 0, st>>>(d_cnt.p, CNT_SLOTS, 0u);
+        LVRHIP_LAUNCH_CHECK();
         if (n == 0) return;
         d_off.reserve(ns + 1);
         d_pstart.reserve(ns + 1);
@@ -755,7 +901,6 @@ struct lvrhip_surface {
         s_list1.reserve(n);
         a_sel.reserve(n);
         a_list.reserve(n);
-        d_cnt.reserve(8);
     }
 
     // calculateSurfaceNormals' adaptive rounds (AdaptiveKSearchSurface.tcc:187-284) for the
@@ -775,7 +920,7 @@ struct lvrhip_surface {
             if (r > 1) prm.tiles_per_wave = std::max(1, tiles_per_wave / 2);
             prm.stats = kstats_ptr(r - 1);
             const uint32_t* qsel = r == 1 ? q0 : lists[(r - 1) & 1];
-            const uint32_t* nq_dev = r == 1 ? nullptr : d_cnt.p + r;
+            const uint32_t* nq_dev = r == 1 ? nullptr : counter(r);
             QueryView qv{px.p, py.p, pz.p, qsel, (uint32_t)nq0, nq_dev};
             auto fill = [&](auto& s) {
                 s.mom = mom.p;
@@ -825,9 +970,9 @@ struct lvrhip_surface {
             plane_fit_kernel<<<(unsigned)ceil_div(nq0, 256), 256, 0, st>>>(pf);
             LVRHIP_LAUNCH_CHECK();
             // rejected queries -> the next round's list and count
-            if (r < 5) compact_flags(s_flags.p, nq0, nq_dev, qsel, lists[r & 1], d_cnt.p + r + 1);
+            if (r < 5) compact_flags(s_flags.p, nq0, nq_dev, qsel, lists[r & 1], counter(r + 1));
         }
-        if (merge_retry) enqueue_merged_rounds(lists[1], d_cnt.p + 2, nq0);
+        if (merge_retry) enqueue_merged_rounds(lists[1], counter(2), nq0);
     }
 
     // Rounds 2..5 for the queries round 1 rejected, in one search with four targets
@@ -856,19 +1001,19 @@ struct lvrhip_surface {
             prof.stop(tok, st);
         }
         plane_fit(qsel, nq0, nq_dev, redo, prm.t0, 5);
-        fill_u32_kernel<<<1, 64, 0, st>>>(d_cnt.p + 3, 3, 0u);
+        fill_u32_kernel<<<1, 64, 0, st>>>(counter(3), 3, 0u);
         LVRHIP_LAUNCH_CHECK();
-        round_count_kernel<<<(unsigned)ceil_div(nq0, 256), 256, 0, st>>>(qsel, nq_dev, nq0, kround.p, d_cnt.p);
+        round_count_kernel<<<(unsigned)ceil_div(nq0, 256), 256, 0, st>>>(qsel, nq_dev, nq0, kround.p, counter(0));
         LVRHIP_LAUNCH_CHECK();
         // points whose search stops at round 2, 3 or 4
         for (int r = 2; r <= 4; r++) {
             select_round_kernel<<<(unsigned)ceil_div(nq0, 256), 256, 0, st>>>(redo, nq0, (uint32_t)r, a_sel.p);
             LVRHIP_LAUNCH_CHECK();
-            compact_flags(a_sel.p, nq0, nq_dev, qsel, a_list.p, d_cnt.p + 6);
+            compact_flags(a_sel.p, nq0, nq_dev, qsel, a_list.p, counter(6));
             const int t0 = clamp_k((int64_t)kn << r);
             TileParams pr = params(t0, 0, 1);
             pr.stats = kstats_ptr(r - 1);
-            QueryView qv{px.p, py.p, pz.p, a_list.p, (uint32_t)nq0, d_cnt.p + 6};
+            QueryView qv{px.p, py.p, pz.p, a_list.p, (uint32_t)nq0, counter(6)};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
                 SinkNormals<P> s{};
@@ -882,7 +1027,7 @@ struct lvrhip_surface {
             tile<SinkNormals>(qv, pr, lanes_per_query, mk(std::integral_constant<int, 2>()),
                               mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
             prof.stop(tok, st);
-            plane_fit(a_list.p, nq0, d_cnt.p + 6, nullptr, t0, r);
+            plane_fit(a_list.p, nq0, counter(6), nullptr, t0, r);
         }
     }
 
@@ -1229,12 +1374,25 @@ struct lvrhip_surface {
         const int tsort = prof.start(LVRHIP_T_SDF_SORT, st);
         sort_queries(d_q, m);
         prof.stop(tsort, st);
+        const bool bf = best_first(t0);
+        const uint32_t* qsel = nullptr;
+        const uint32_t* nq_dev = nullptr;
+        if (bf) {
+            // best-first for every vertex; the tile kernel redoes the ones whose stacks overflowed
+            s_redo.reserve(m);
+            BfSdfOut o{px.p, py.p, pz.p, nx.p, ny.p, nz.p, s_v0.p, voxel, d_proj, d_eucl, d_inv, s_redo.p};
+            const int tok = prof.start(LVRHIP_T_SDF_KERNEL, st);
+            enqueue_bf(m, t0, BF_EMPTY, o, 6);
+            prof.stop(tok, st);
+            qsel = s_redo_list.p;
+            nq_dev = counter(8);
+        }
         TileParams prm = params(t0, 0, 2);
         prm.tiles_per_wave = sdf_tiles_per_wave;
         prm.r2_margin = sdf_r2_margin;
         prm.grow_min = sdf_grow_min;
         prm.stats = kstats_ptr(6);
-        QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
+        QueryView qv{s_qx.p, s_qy.p, s_qz.p, qsel, (uint32_t)m, nq_dev};
         auto mk = [&](auto tag) {
             constexpr int P = decltype(tag)::value;
             SinkSdf<P> s{};
@@ -1256,7 +1414,7 @@ struct lvrhip_surface {
             s.npts = (uint32_t)npad;
             return s;
         };
-        const int tok = prof.start(LVRHIP_T_SDF_KERNEL, st);
+        const int tok = prof.start(bf ? LVRHIP_T_SDF_REDO : LVRHIP_T_SDF_KERNEL, st);
         tile<SinkSdf>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
                       mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
         prof.stop(tok, st);
@@ -1284,11 +1442,11 @@ struct lvrhip_surface {
         g_idx.reserve(m);
         lat_first_flags_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(keys, m, g_flags.p);
         LVRHIP_LAUNCH_CHECK();
-        compact_flags(g_flags.p, m, nullptr, nullptr, g_idx.p, d_cnt.p + 7);
-        lat_gather_keys_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(keys, g_idx.p, d_cnt.p + 7, out);
+        compact_flags(g_flags.p, m, nullptr, nullptr, g_idx.p, counter(7));
+        lat_gather_keys_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(keys, g_idx.p, counter(7), out);
         LVRHIP_LAUNCH_CHECK();
         uint32_t c = 0;
-        LVRHIP_CHECK(hipMemcpyAsync(&c, d_cnt.p + 7, sizeof(c), hipMemcpyDeviceToHost, st));
+        LVRHIP_CHECK(hipMemcpyAsync(&c, counter(7), sizeof(c), hipMemcpyDeviceToHost, st));
         LVRHIP_CHECK(hipStreamSynchronize(st));
         return c;
     }
@@ -1300,7 +1458,6 @@ struct lvrhip_surface {
         drop_graph();
         grid_nv = grid_nc = 0;
         if (n == 0) return;
-        d_cnt.reserve(8);
         // the cloud's box (the surface's bounding box, PointsetSurface.tcc:52-56)
         uint32_t enc[6];
         LVRHIP_CHECK(hipMemcpyAsync(enc, sbox.p, sizeof(enc), hipMemcpyDeviceToHost, st));
@@ -1363,13 +1520,13 @@ struct lvrhip_surface {
         s_bsum.reserve(nb + 1);
         flag_block_sums_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(g_flags.p, m, nullptr, s_bsum.p);
         LVRHIP_LAUNCH_CHECK();
-        scan_exclusive_kernel<<<1, 1024, 0, st>>>(s_bsum.p, nb, d_cnt.p + 7);
+        scan_exclusive_kernel<<<1, 1024, 0, st>>>(s_bsum.p, nb, counter(7));
         LVRHIP_LAUNCH_CHECK();
         g_gid.reserve(m);
         lat_group_ids_kernel<<<(unsigned)nb, LAT_THREADS, 0, st>>>(g_flags.p, m, s_bsum.p, g_gid.p);
         LVRHIP_LAUNCH_CHECK();
         uint32_t nvu = 0;
-        LVRHIP_CHECK(hipMemcpyAsync(&nvu, d_cnt.p + 7, sizeof(nvu), hipMemcpyDeviceToHost, st));
+        LVRHIP_CHECK(hipMemcpyAsync(&nvu, counter(7), sizeof(nvu), hipMemcpyDeviceToHost, st));
         LVRHIP_CHECK(hipStreamSynchronize(st));
         const int64_t nv = nvu;
         // vertex ids in first-use order
@@ -1417,7 +1574,6 @@ struct lvrhip_surface {
         a_flags.reserve(R);
         a_sel.reserve(R);
         a_list.reserve(R);
-        d_cnt.reserve(8);
         zero_f32_kernel<<<(unsigned)ceil_div(3 * R, 256), 256, 0, st>>>(g_p.p, 3 * R);
         LVRHIP_LAUNCH_CHECK();
         zero_f32_kernel<<<(unsigned)ceil_div(3 * R, 256), 256, 0, st>>>(g_n.p, 3 * R);
@@ -1467,10 +1623,10 @@ struct lvrhip_surface {
         for (int r = 1; r <= 5; r++) {
             select_round_kernel<<<(unsigned)ceil_div(R, 256), 256, 0, st>>>(a_flags.p, R, (uint32_t)(1 + r), a_sel.p);
             LVRHIP_LAUNCH_CHECK();
-            compact_flags(a_sel.p, R, nullptr, nullptr, a_list.p, d_cnt.p + 6);
+            compact_flags(a_sel.p, R, nullptr, nullptr, a_list.p, counter(6));
             TileParams prm = params(clamp_k((int64_t)kn << r), 0, 3);
             prm.stats = kstats_ptr(7);
-            QueryView qv{px.p, py.p, pz.p, a_list.p, (uint32_t)R, d_cnt.p + 6};
+            QueryView qv{px.p, py.p, pz.p, a_list.p, (uint32_t)R, counter(6)};
             auto mk = [&](auto tag) {
                 constexpr int P = decltype(tag)::value;
                 SinkPcaAdjoint<P> s{};
@@ -1490,37 +1646,139 @@ struct lvrhip_surface {
         prof.collect();
     }
 
-    void knn_device(const float* d_q, int64_t m, int k, int32_t* d_idx, float* d_d2)
+    bool best_first(int t) const { return search_mode != LVRHIP_SEARCH_TILE && t >= 1 && t <= BF_KMAX; }
+
+    // Best-first search of the m sorted queries s_q*: one lane per query for all of them, then one
+    // wave per query for those whose lane stack overflowed; the queries both left (redo flags
+    // set by Out) end up in s_redo_list[0, *counter(8)), for the tile kernel.  Enqueue only.
+    template <class Out>
+    void enqueue_bf(int64_t m, int t0, uint64_t empty, const Out& o, int stats_group)
+    {
+        s_redo.reserve(m);
+        s_redo_list.reserve(m);
+        if (search_mode == LVRHIP_SEARCH_WAVE) {
+            fill_u32_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(s_redo.p, m, 1u);
+            LVRHIP_LAUNCH_CHECK();
+        } else {
+            uint32_t* q = nullptr;
+            if (use_queue) {
+                d_queue.reserve(QUEUE_SEGS * QUEUE_STRIDE);
+                q = d_queue.p;
+            }
+            // one spill area per wave of the grid
+            const int64_t waves = q ? std::min(ceil_div(m, BF_THREADS), bf_resident) : ceil_div(m, BF_THREADS);
+            s_spill.reserve((size_t)waves * BF_SPILL * BF_THREADS);
+            launch_bf(view(), s_qx.p, s_qy.p, s_qz.p, m, t0, empty, o, st, kstats_ptr(stats_group), q, bf_resident,
+                      s_spill.p);
+        }
+        compact_flags(s_redo.p, m, nullptr, nullptr, s_redo_list.p, counter(8));
+        launch_bf_wave(view(), s_qx.p, s_qy.p, s_qz.p, s_redo_list.p, counter(8), t0, empty, o, st,
+                       kstats_ptr(stats_group == 6 ? 5 : stats_group));
+        compact_flags(s_redo.p, m, nullptr, nullptr, s_redo_list.p, counter(8));
+    }
+
+    // Number of queries the last enqueue_bf left to the tile kernel (synchronises).
+    uint32_t bf_redo_count()
+    {
+        uint32_t c = 0;
+        LVRHIP_CHECK(hipMemcpyAsync(&c, counter(8), sizeof(c), hipMemcpyDeviceToHost, st));
+        LVRHIP_CHECK(hipStreamSynchronize(st));
+        return c;
+    }
+
+    // Sorted kSearch rows (input index, d^2; padded with -1 / +inf) for m device queries.  With
+    // r2 >= 0 (radiusSearch) a row keeps only the neighbours with d^2 < r2, and d_count[row] is
+    // their number.
+    void knn_device(const float* d_q, int64_t m, int k, int32_t* d_idx, float* d_d2, float r2 = -1.0f,
+                    int32_t* d_count = nullptr)
     {
         require_one_scan("kSearch");
         if (m == 0) return;
         drop_graph();
         const int t0 = (int)std::min<int64_t>(k, n);
-        if (t0 > 0) {
-            sort_queries(d_q, m);
-            DBuf<uint64_t> keys;
-            keys.reserve((size_t)m * k);
-            TileParams prm = params(t0, 0, 3);
-            prm.stats = kstats_ptr(7);
-            QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
-            auto mk = [&](auto tag) {
-                constexpr int P = decltype(tag)::value;
-                SinkList<P> s{};
-                s.out = keys.p;
-                s.qorig = s_v0.p;
-                s.kw = k;
-                return s;
-            };
-            tile<SinkList>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
-                           mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
-            sort_rows_kernel<<<(unsigned)m, kWave, 0, st>>>(keys.p, m, k, t0, d_idx, d_d2);
-            LVRHIP_LAUNCH_CHECK();
-            check_err();
-        } else {
+        if (t0 == 0) {
             sort_rows_kernel<<<(unsigned)m, kWave, 0, st>>>(nullptr, m, k, 0, d_idx, d_d2);
             LVRHIP_LAUNCH_CHECK();
+            if (d_count) LVRHIP_CHECK(hipMemsetAsync(d_count, 0, (size_t)m * sizeof(int32_t), st));
             LVRHIP_CHECK(hipStreamSynchronize(st));
+            return;
         }
+        sort_queries(d_q, m);
+        if (best_first(t0)) {
+            s_redo.reserve(m);
+            const uint64_t empty = r2 >= 0.0f ? ((uint64_t)host_fbits(r2) << 32) : BF_EMPTY;
+            BfRowsOut o{s_v0.p, k, d_idx, d_d2, d_count, s_redo.p};
+            enqueue_bf(m, t0, empty, o, 7);
+            if (bf_redo_count() == 0) {
+                check_err();
+                return;
+            }
+        }
+        // the tile kernel for every query (also when a best-first stack overflowed)
+        DBuf<uint64_t> keys;
+        keys.reserve((size_t)m * k);
+        DBuf<float> d2tmp;
+        if (r2 >= 0.0f && !d_d2) {
+            d2tmp.reserve((size_t)m * k);
+            d_d2 = d2tmp.p;
+        }
+        TileParams prm = params(t0, 0, 3);
+        prm.stats = kstats_ptr(7);
+        QueryView qv{s_qx.p, s_qy.p, s_qz.p, nullptr, (uint32_t)m, nullptr};
+        auto mk = [&](auto tag) {
+            constexpr int P = decltype(tag)::value;
+            SinkList<P> s{};
+            s.out = keys.p;
+            s.qorig = s_v0.p;
+            s.kw = k;
+            return s;
+        };
+        tile<SinkList>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
+                       mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+        sort_rows_kernel<<<(unsigned)m, kWave, 0, st>>>(keys.p, m, k, t0, d_idx, d_d2);
+        LVRHIP_LAUNCH_CHECK();
+        if (r2 >= 0.0f) {
+            radius_filter_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(m, k, r2, d_idx, d_d2, d_count);
+            LVRHIP_LAUNCH_CHECK();
+        }
+        check_err();
+    }
+
+    // PointsetSurface::getInterpolatedNormal (PointsetSurface.tcc:61-74) at m device positions.
+    void interpolated_normals_device(const float* d_q, int64_t m, float* d_out)
+    {
+        if (!has_normals) throw Error(LVRHIP_ESTATE, "interpolated normals requested before calculate_normals");
+        require_one_scan("getInterpolatedNormal");
+        if (ki < 1) throw Error(LVRHIP_EINVAL, "getInterpolatedNormal needs ki >= 1");
+        if (m == 0) return;
+        if (n == 0) throw Error(LVRHIP_ESTATE, "getInterpolatedNormal on an empty point cloud");
+        drop_graph();
+        const int t0 = clamp_k(ki);
+        if (best_first(t0)) {
+            sort_queries(d_q, m);
+            s_redo.reserve(m);
+            BfMeanNormalOut o{nx.p, ny.p, nz.p, s_v0.p, ki, d_out, s_redo.p};
+            enqueue_bf(m, t0, BF_EMPTY, o, 7);
+            if (bf_redo_count() == 0) {
+                check_err();
+                return;
+            }
+        }
+        DBuf<int32_t> idx;
+        idx.reserve((size_t)m * t0);
+        const int saved = search_mode;
+        search_mode = LVRHIP_SEARCH_TILE;
+        try {
+            knn_device(d_q, m, t0, idx.p, nullptr);
+        } catch (...) {
+            search_mode = saved;
+            throw;
+        }
+        search_mode = saved;
+        mean_normal_rows_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(m, t0, t0, idx.p, rank.p, nx.p, ny.p,
+                                                                          nz.p, ki, d_out);
+        LVRHIP_LAUNCH_CHECK();
+        LVRHIP_CHECK(hipStreamSynchronize(st));
     }
 };
 
@@ -1587,7 +1845,11 @@ int lvrhip_create_scans_device(const float* d_xyz, const int64_t* scan_offsets, 
         try {
             int cus = 0;
             if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+            {
                 h->resident_waves = (int64_t)cus * 16;
+                // 12 best-first waves per CU (3 per SIMD: their registers; LDS would allow 20)
+                h->bf_resident = (int64_t)cus * 12;
+            }
             h->init(d_xyz, scan_offsets, nscans);
             h->build();
             LVRHIP_CHECK(hipStreamSynchronize(h->st));
@@ -1788,6 +2050,65 @@ int lvrhip_knn(lvrhip_t h, const float* q, size_t m, int k, int32_t* out_idx, fl
         if (out_d2)
             LVRHIP_CHECK(hipMemcpyAsync(out_d2, dd.p, m * (size_t)k * sizeof(float), hipMemcpyDeviceToHost, h->st));
         LVRHIP_CHECK(hipStreamSynchronize(h->st));
+    });
+}
+
+int lvrhip_radius_search(lvrhip_t h, const float* q, size_t m, int k, float r, int32_t* out_idx, float* out_d2,
+                         int32_t* out_count)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        REQUIRE(k >= 1 && k <= (1 << 16), "k out of range");
+        REQUIRE(r >= 0.0f, "radius must be a non-negative number");
+        REQUIRE(m == 0 || (q && out_idx), "NULL query or output pointer");
+        REQUIRE(m < (size_t)0x7FFFFFFF, "too many queries");
+        if (m == 0) return;
+        DeviceGuard dg(h->device);
+        DBuf<float> dq, dd;
+        DBuf<int32_t> di, dc;
+        dq.reserve(3 * m);
+        di.reserve(m * (size_t)k);
+        dd.reserve(m * (size_t)k);
+        dc.reserve(m);
+        LVRHIP_CHECK(hipMemcpyAsync(dq.p, q, 3 * m * sizeof(float), hipMemcpyHostToDevice, h->st));
+        h->knn_device(dq.p, (int64_t)m, k, di.p, dd.p, r * r, dc.p);
+        LVRHIP_CHECK(hipMemcpyAsync(out_idx, di.p, m * (size_t)k * sizeof(int32_t), hipMemcpyDeviceToHost, h->st));
+        if (out_d2)
+            LVRHIP_CHECK(hipMemcpyAsync(out_d2, dd.p, m * (size_t)k * sizeof(float), hipMemcpyDeviceToHost, h->st));
+        if (out_count)
+            LVRHIP_CHECK(hipMemcpyAsync(out_count, dc.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, h->st));
+        LVRHIP_CHECK(hipStreamSynchronize(h->st));
+    });
+}
+
+int lvrhip_interpolated_normals(lvrhip_t h, const float* q, size_t m, float* out_xyz)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        REQUIRE(m == 0 || (q && out_xyz), "NULL query or output pointer");
+        REQUIRE(m < (size_t)0x7FFFFFFF, "too many queries");
+        DeviceGuard dg(h->device);
+        if (m == 0) {
+            h->interpolated_normals_device(nullptr, 0, nullptr);
+            return;
+        }
+        DBuf<float> dq, dn;
+        dq.reserve(3 * m);
+        dn.reserve(3 * m);
+        LVRHIP_CHECK(hipMemcpyAsync(dq.p, q, 3 * m * sizeof(float), hipMemcpyHostToDevice, h->st));
+        h->interpolated_normals_device(dq.p, (int64_t)m, dn.p);
+        LVRHIP_CHECK(hipMemcpyAsync(out_xyz, dn.p, 3 * m * sizeof(float), hipMemcpyDeviceToHost, h->st));
+        LVRHIP_CHECK(hipStreamSynchronize(h->st));
+    });
+}
+
+int lvrhip_set_search_mode(lvrhip_t h, int mode)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        REQUIRE(mode == LVRHIP_SEARCH_AUTO || mode == LVRHIP_SEARCH_TILE || mode == LVRHIP_SEARCH_WAVE,
+                "search mode must be 0, 1 or 2");
+        h->search_mode = mode;
     });
 }
 

@@ -121,6 +121,250 @@ inline double hit_room(const double* o, const double* d, const Box& room)
     return best;
 }
 
+// ---- outdoor scene: Perlin heightfield + box buildings ------------------------------------
+
+// 2D gradient noise (Perlin's improved noise, quintic fade) over a permutation of 0..255 drawn
+// from the seed; eight unit gradient directions.
+struct Perlin {
+    int perm[512];
+    explicit Perlin(uint64_t seed)
+    {
+        Rng g(seed, 0x9E71ull);
+        int p[256];
+        for (int i = 0; i < 256; i++) p[i] = i;
+        for (int i = 255; i > 0; i--) std::swap(p[i], p[(int)(g.next() % (uint64_t)(i + 1))]);
+        for (int i = 0; i < 512; i++) perm[i] = p[i & 255];
+    }
+    static double fade(double t) { return t * t * t * (t * (t * 6 - 15) + 10); }
+    static double grad(int h, double x, double y)
+    {
+        static const double c = 0.70710678118654752;
+        static const double gx[8] = {1, -1, 0, 0, c, -c, c, -c}, gy[8] = {0, 0, 1, -1, c, c, -c, -c};
+        return gx[h & 7] * x + gy[h & 7] * y;
+    }
+    double noise(double x, double y) const
+    {
+        const double fx = std::floor(x), fy = std::floor(y);
+        const int xi = (int)((int64_t)fx & 255), yi = (int)((int64_t)fy & 255);
+        const double xf = x - fx, yf = y - fy;
+        const double u = fade(xf), v = fade(yf);
+        const int a = perm[xi] + yi, b = perm[xi + 1] + yi;
+        const double n00 = grad(perm[a], xf, yf), n10 = grad(perm[b], xf - 1, yf);
+        const double n01 = grad(perm[a + 1], xf, yf - 1), n11 = grad(perm[b + 1], xf - 1, yf - 1);
+        const double x0 = n00 + u * (n10 - n00), x1 = n01 + u * (n11 - n01);
+        return x0 + v * (x1 - x0);
+    }
+    // fractal sum: 5 octaves from a 128 m wavelength, gain 1/2
+    double fbm(double x, double y) const
+    {
+        double s = 0, amp = 1, f = 1.0 / 128.0;
+        for (int o = 0; o < 5; o++) {
+            s += amp * noise(x * f + 17.3 * o, y * f - 9.1 * o);
+            amp *= 0.5;
+            f *= 2;
+        }
+        return s;
+    }
+};
+
+// A 200 x 200 m area centred on the origin: terrain z = H(x, y), bilinear over a 0.5 m grid of
+// Perlin fBm samples (+-3 m), and box buildings standing in it (tops at most 25 m above the
+// terrain's lowest point, so the scene is at most 30 m high).
+struct Outdoor {
+    static constexpr double HALF = 100.0, CELL = 0.5;
+    static constexpr int G = 400;  // cells per side
+    std::vector<double> h;         // (G + 1)^2 samples
+    double slope = 0;              // largest |dH/dx| + |dH/dy| over the bilinear patches
+    std::vector<Box> blds;
+
+    double at(int i, int j) const { return h[(size_t)j * (G + 1) + i]; }
+    double height(double x, double y) const
+    {
+        double u = (x + HALF) / CELL, v = (y + HALF) / CELL;
+        u = std::min(std::max(u, 0.0), (double)G);
+        v = std::min(std::max(v, 0.0), (double)G);
+        int i = std::min((int)u, G - 1), j = std::min((int)v, G - 1);
+        const double a = u - i, b = v - j;
+        const double h0 = at(i, j) + a * (at(i + 1, j) - at(i, j));
+        const double h1 = at(i, j + 1) + a * (at(i + 1, j + 1) - at(i, j + 1));
+        return h0 + b * (h1 - h0);
+    }
+    bool in_building(double x, double y, double margin) const
+    {
+        for (auto& b : blds)
+            if (x > b.lo[0] - margin && x < b.hi[0] + margin && y > b.lo[1] - margin && y < b.hi[1] + margin)
+                return true;
+        return false;
+    }
+};
+
+static Outdoor make_outdoor(uint64_t seed, int n_bld)
+{
+    Outdoor sc;
+    const Perlin pn(seed);
+    const int G = Outdoor::G;
+    sc.h.resize((size_t)(G + 1) * (G + 1));
+    for (int j = 0; j <= G; j++)
+        for (int i = 0; i <= G; i++)
+            sc.h[(size_t)j * (G + 1) + i] =
+                3.0 * pn.fbm(-Outdoor::HALF + i * Outdoor::CELL, -Outdoor::HALF + j * Outdoor::CELL);
+    double zmin = INFINITY;
+    for (double v : sc.h) zmin = std::min(zmin, v);
+    for (int j = 0; j < G; j++)
+        for (int i = 0; i < G; i++) {
+            const double sx = std::max(std::fabs(sc.at(i + 1, j) - sc.at(i, j)),
+                                       std::fabs(sc.at(i + 1, j + 1) - sc.at(i, j + 1)));
+            const double sy = std::max(std::fabs(sc.at(i, j + 1) - sc.at(i, j)),
+                                       std::fabs(sc.at(i + 1, j + 1) - sc.at(i + 1, j)));
+            sc.slope = std::max(sc.slope, (sx + sy) / Outdoor::CELL);
+        }
+    Rng g(seed, 0xB1D6ull);
+    for (int b = 0; b < n_bld; b++) {
+        for (int tries = 0; tries < 1000; tries++) {
+            const double sx = g.uniform(2, 8), sy = g.uniform(2, 8), ht = g.uniform(3, 20);
+            const double cx = g.uniform(-Outdoor::HALF + sx, Outdoor::HALF - sx);
+            const double cy = g.uniform(-Outdoor::HALF + sy, Outdoor::HALF - sy);
+            if (std::hypot(cx, cy) < 0.5 * std::hypot(sx, sy) + 5.0) continue;  // the central scanner
+            double base = INFINITY;
+            for (int c = 0; c < 4; c++)
+                base = std::min(base, sc.height(cx + ((c & 1) ? sx : -sx) / 2, cy + ((c & 2) ? sy : -sy) / 2));
+            base -= 0.5;  // set into the ground
+            const double top = std::min(base + ht, zmin + 25.0);
+            if (top <= base + 1.0) continue;
+            sc.blds.push_back(Box{{cx - sx / 2, cy - sy / 2, base}, {cx + sx / 2, cy + sy / 2, top}});
+            break;
+        }
+    }
+    return sc;
+}
+
+// Building lists per azimuth bin as seen from o: a ray tests only the buildings of its bin.
+struct AzBins {
+    static constexpr int NB = 7200;
+    std::vector<std::vector<int>> bins;
+    AzBins(const Outdoor& sc, const double* o) : bins(NB)
+    {
+        for (int b = 0; b < (int)sc.blds.size(); b++) {
+            const Box& B = sc.blds[b];
+            double a[4];
+            for (int c = 0; c < 4; c++)
+                a[c] = std::atan2(((c & 2) ? B.hi[1] : B.lo[1]) - o[1], ((c & 1) ? B.hi[0] : B.lo[0]) - o[0]);
+            // the smallest arc covering the four corner directions (o is outside the footprint)
+            double lo = a[0], width = 0;
+            for (int c = 0; c < 4; c++) {
+                double w = 0;
+                for (int e = 0; e < 4; e++) {
+                    double d = a[e] - a[c];
+                    while (d < 0) d += 2 * M_PI;
+                    w = std::max(w, d);
+                }
+                if (c == 0 || w < width) {
+                    width = w;
+                    lo = a[c];
+                }
+            }
+            const int b0 = (int)std::floor((lo + M_PI) / (2 * M_PI) * NB) - 1;
+            const int nb = (int)std::ceil(width / (2 * M_PI) * NB) + 3;
+            for (int k = 0; k < nb && k < NB; k++) bins[(size_t)(((b0 + k) % NB) + NB) % NB].push_back(b);
+        }
+    }
+    const std::vector<int>& of(double az) const
+    {
+        int k = (int)std::floor((az + M_PI) / (2 * M_PI) * NB);
+        return bins[(size_t)((k % NB) + NB) % NB];
+    }
+};
+
+// First hit of the ray o + t d (|d| = 1) with the terrain or a building, or +inf.
+static double hit_outdoor(const Outdoor& sc, const AzBins& az, const double* o, const double* d)
+{
+    double tb = INFINITY;
+    for (int b : az.of(std::atan2(d[1], d[0]))) tb = std::min(tb, hit_box(o, d, sc.blds[b]));
+    // leave the area: the terrain exists inside the 200 x 200 m square only
+    double tx = INFINITY;
+    for (int a = 0; a < 2; a++) {
+        if (d[a] > 1e-15) tx = std::min(tx, (Outdoor::HALF - o[a]) / d[a]);
+        if (d[a] < -1e-15) tx = std::min(tx, (-Outdoor::HALF - o[a]) / d[a]);
+    }
+    const double tmax = std::min(tb, tx);
+    // terrain: f(t) = height above ground along the ray; |f'| <= |dz| + slope * |dxy|, so steps of
+    // f / that bound never cross the surface
+    const double lip = std::fabs(d[2]) + sc.slope * std::hypot(d[0], d[1]);
+    double t = 0;
+    for (int it = 0; it < 100000 && t < tmax; it++) {
+        const double f = o[2] + t * d[2] - sc.height(o[0] + t * d[0], o[1] + t * d[1]);
+        if (f < 1e-5) return t;
+        t += f / lip;
+    }
+    return tb;
+}
+
+// Exactly n points from rays on a uniform (azimuth 0..360, elevation -60..+90 deg) grid from o:
+// the grid is sized from a pilot scan to give at least n hits, and the hits, in ray order, are
+// thinned evenly to n.  Radial N(0, sigma^2) noise from the ray's own stream.
+static int cast_outdoor(const Outdoor& sc, const double* o, int64_t n, double sigma, uint64_t noise_seed, float* out)
+{
+    if (n <= 0) return 0;
+    const AzBins az(sc, o);
+    auto ray = [](int64_t i, int n_az, int n_el, double* d) {
+        const int ia = (int)(i / n_el), ie = (int)(i % n_el);
+        const double a = (ia + 0.5) * (2 * M_PI / n_az);
+        const double el = (-60.0 + (ie + 0.5) * (150.0 / n_el)) * (M_PI / 180.0);
+        d[0] = std::cos(el) * std::cos(a);
+        d[1] = std::cos(el) * std::sin(a);
+        d[2] = std::sin(el);
+    };
+    // hit fraction on a pilot grid
+    const int pe = 64, pa = 164;
+    int64_t ph = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : ph)
+    for (int64_t i = 0; i < (int64_t)pa * pe; i++) {
+        double d[3];
+        ray(i, pa, pe, d);
+        if (std::isfinite(hit_outdoor(sc, az, o, d))) ph++;
+    }
+    double frac = std::max((double)ph / ((double)pa * pe), 1e-3);
+    for (int attempt = 0; attempt < 8; attempt++) {
+        const double rays = (double)n / frac * 1.03 + 64;
+        const int n_el = std::max(4, (int)std::llround(std::sqrt(rays / 2.56)));
+        const int n_az = std::max(4, (int)std::ceil(rays / n_el));
+        const int64_t total = (int64_t)n_az * n_el;
+        std::vector<float> tt((size_t)total);
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t i = 0; i < total; i++) {
+            double d[3];
+            ray(i, n_az, n_el, d);
+            tt[(size_t)i] = (float)hit_outdoor(sc, az, o, d);
+        }
+        // hits in ray order: their ranks
+        std::vector<int64_t> rank((size_t)total);
+        int64_t hits = 0;
+        for (int64_t i = 0; i < total; i++) {
+            rank[(size_t)i] = hits;
+            if (std::isfinite(tt[(size_t)i])) hits++;
+        }
+        if (hits < n) {
+            frac = std::max((double)hits / (double)total, 1e-3) * 0.97;
+            continue;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < total; i++) {
+            if (!std::isfinite(tt[(size_t)i])) continue;
+            const int64_t j = rank[(size_t)i];
+            // hit j is kept iff the even thinning hits -> n advances at it
+            const int64_t k0 = (int64_t)((__int128)j * n / hits), k1 = (int64_t)((__int128)(j + 1) * n / hits);
+            if (k1 == k0) continue;
+            double d[3];
+            ray(i, n_az, n_el, d);
+            Rng r(noise_seed, (uint64_t)i);
+            const double t = (double)tt[(size_t)i] + sigma * r.gauss();
+            for (int c = 0; c < 3; c++) out[3 * k0 + c] = (float)(o[c] + d[c] * t);
+        }
+        return 0;
+    }
+    return 1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -282,6 +526,41 @@ int scene_room_scan_at(uint64_t seed, int n_az, int n_el, int n_obj, double sigm
     const Room rm = make_room(seed, n_obj);
     cast_room(rm, noise_seed, n_az, n_el, sigma, pos, out);
     return 0;
+}
+
+// The outdoor scene of `seed` with n_bld buildings scanned from its centre (1.8 m above the
+// terrain at the origin): exactly n points, radial noise sigma.
+int scene_outdoor_scan(uint64_t seed, int n_bld, int64_t n, double sigma, float* out, double* scanner)
+{
+    const Outdoor sc = make_outdoor(seed, n_bld);
+    const double o[3] = {0.0, 0.0, sc.height(0.0, 0.0) + 1.8};
+    if (scanner) std::memcpy(scanner, o, sizeof(o));
+    return cast_outdoor(sc, o, n, sigma, seed, out);
+}
+
+// A scanner pose in the outdoor scene: (x, y) uniform within 90 m of the centre and at least 2 m
+// from every building, 1.8 m above the terrain (pose_seed picks it).
+int scene_outdoor_pose(uint64_t seed, int n_bld, uint64_t pose_seed, double* out)
+{
+    const Outdoor sc = make_outdoor(seed, n_bld);
+    Rng g(pose_seed, 0x9053ull);
+    for (int tries = 0; tries < 100000; tries++) {
+        const double x = g.uniform(-90.0, 90.0), y = g.uniform(-90.0, 90.0);
+        if (sc.in_building(x, y, 2.0)) continue;
+        out[0] = x;
+        out[1] = y;
+        out[2] = sc.height(x, y) + 1.8;
+        return 0;
+    }
+    return 1;
+}
+
+// The outdoor scene scanned from `pos`: exactly n points, noise drawn from noise_seed.
+int scene_outdoor_scan_at(uint64_t seed, int n_bld, const double* pos, int64_t n, double sigma, uint64_t noise_seed,
+                          float* out)
+{
+    const Outdoor sc = make_outdoor(seed, n_bld);
+    return cast_outdoor(sc, pos, n, sigma, noise_seed, out);
 }
 
 // Noisy sphere + torus: a unit-radius-scaled sphere of radius R at the origin and a

@@ -432,6 +432,35 @@ __global__ __launch_bounds__(256) void plane_fit_kernel(PlaneFitParams a)
     if (a.kround && a.kcount >= 0) a.kround[qpos] = (uint8_t)a.round;
 }
 
+// AdaptiveKSearchSurface::distance (tcc:435-457) from the neighbours' means, with the invalid
+// rule of PointsetGrid::calcDistanceValues (PointsetGrid.tcc:138-143): shared by the tile
+// kernel's SDF sink and the best-first search.
+__device__ __forceinline__ void sdf_emit(float ax, float ay, float az, float bx, float by, float bz, int cnt, float qx,
+                                         float qy, float qz, float voxel, uint32_t oi, float* out_proj,
+                                         float* out_eucl, uint8_t* out_invalid)
+{
+    const float c = (float)cnt;
+    bx /= c;
+    by /= c;
+    bz /= c;
+    ax /= c;
+    ay /= c;
+    az /= c;
+    if (!(bx == 0.0f && by == 0.0f && bz == 0.0f)) {  // BaseVector::normalize
+        const float l = sqrtf(bx * bx + by * by + bz * bz);
+        bx /= l;
+        by /= l;
+        bz /= l;
+    }
+    const float dx = qx - ax, dy = qy - ay, dz = qz - az;
+    const float proj = dx * bx + dy * by + dz * bz;
+    const float eucl = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (out_proj) out_proj[oi] = proj;
+    if (out_eucl) out_eucl[oi] = eucl;
+    // the comparison is made in double
+    if (out_invalid) out_invalid[oi] = ((double)eucl > 1.75 * (double)voxel) ? 1 : 0;
+}
+
 template <int P>
 struct SinkSdf {
     static constexpr int kTargets = 2;
@@ -470,22 +499,8 @@ struct SinkSdf {
     __device__ __forceinline__ void emit(float ax, float ay, float az, float bx, float by, float bz, int cnt,
                                          uint32_t qpos, float qx, float qy, float qz)
     {
-        const float c = (float)cnt;
-        bx /= c;
-        by /= c;
-        bz /= c;
-        ax /= c;
-        ay /= c;
-        az /= c;
-        normalize3(bx, by, bz);
-        const float dx = qx - ax, dy = qy - ay, dz = qz - az;
-        const float proj = dx * bx + dy * by + dz * bz;
-        const float eucl = sqrtf(dx * dx + dy * dy + dz * dz);
-        const uint32_t oi = qorig ? qorig[qpos] : qpos;
-        if (out_proj) out_proj[oi] = proj;
-        if (out_eucl) out_eucl[oi] = eucl;
-        // the comparison is made in double
-        if (out_invalid) out_invalid[oi] = ((double)eucl > 1.75 * (double)voxel) ? 1 : 0;
+        sdf_emit(ax, ay, az, bx, by, bz, cnt, qx, qy, qz, voxel, qorig ? qorig[qpos] : qpos, out_proj, out_eucl,
+                 out_invalid);
     }
 
     __device__ __forceinline__ void finish(int g, int s, bool ok, uint32_t qpos, float qx, float qy, float qz,

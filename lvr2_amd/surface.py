@@ -28,6 +28,24 @@ def _is_device_tensor(a) -> bool:
     return hasattr(a, "is_cuda") and bool(getattr(a, "is_cuda"))
 
 
+def _device_points(t, what: str):
+    """A (M, 3) float32 contiguous view of a device tensor (converted if needed)."""
+    import torch
+
+    if t.ndim != 2 or t.shape[1] != 3:
+        raise ValueError(f"{what} must have shape (M, 3)")
+    return t.detach().to(torch.float32).contiguous()
+
+
+def _check_out(t, shape, dtype, device, what: str) -> None:
+    """An output tensor the library writes through its data pointer: exact shape and dtype,
+    contiguous, on the surface's device."""
+    if not _is_device_tensor(t):
+        raise ValueError(f"{what} must be a device tensor")
+    if tuple(t.shape) != tuple(shape) or t.dtype != dtype or not t.is_contiguous() or t.device != device:
+        raise ValueError(f"{what} must be a contiguous {dtype} tensor of shape {tuple(shape)} on {device}")
+
+
 def _scan_table(sizes) -> np.ndarray:
     sz = np.asarray(sizes, dtype=np.int64).ravel()
     if sz.size < 1 or (sz < 0).any():
@@ -151,9 +169,20 @@ class HipSurface:
     def getNormals(self, out=None):
         """Normals in input order.  ``out`` may be a host array or a device tensor (N, 3)."""
         if out is not None and _is_device_tensor(out):
+            import torch
+
+            _check_out(out, (self._n, 3), torch.float32, out.device, "out")
+            if out.device.index is not None and out.device.index != self.device:
+                raise ValueError("out must be on the surface's device")
             check(lib().lvrhip_get_normals_device(self._h, C.c_void_p(out.data_ptr())))
             return out
-        res = np.empty((self._n, 3), np.float32) if out is None else out
+        if out is None:
+            res = np.empty((self._n, 3), np.float32)
+        else:
+            res = out
+            if not (isinstance(res, np.ndarray) and res.dtype == np.float32 and res.shape == (self._n, 3)
+                    and res.flags["C_CONTIGUOUS"] and res.flags["WRITEABLE"]):
+                raise ValueError("out must be a writeable C-contiguous float32 array of shape (N, 3)")
         check(lib().lvrhip_get_normals(self._h, res.ctypes.data_as(C.c_void_p)))
         return res
 
@@ -208,12 +237,16 @@ class HipSurface:
 
         if not (_is_device_tensor(queries) and _is_device_tensor(grad_proj)):
             raise ValueError("queries and grad_proj must be device tensors")
-        q = queries.detach().to(torch.float32).contiguous()
+        q = _device_points(queries, "queries")
         w = grad_proj.detach().to(torch.float32).contiguous()
-        if w.shape != (q.shape[0],):
+        if tuple(w.shape) != (q.shape[0],):
             raise ValueError("one weight per query")
+        if w.device != q.device:
+            raise ValueError("queries and grad_proj must be on the same device")
         if grad_points is None:
             grad_points = torch.zeros((self._n, 3), dtype=torch.float32, device=q.device)
+        else:
+            _check_out(grad_points, (self._n, 3), torch.float32, q.device, "grad_points")
         check(lib().lvrhip_distances_backward_device(self._h, C.c_void_p(q.data_ptr()), int(q.shape[0]),
                                                      C.c_void_p(w.data_ptr()), C.c_void_p(grad_points.data_ptr())))
         return grad_points
@@ -283,6 +316,33 @@ class HipSurface:
                                idx.ctypes.data_as(C.c_void_p), d2.ctypes.data_as(C.c_void_p)))
         return idx, d2
 
+    def radiusSearch(self, queries, k: int, r: float):
+        """SearchTree::radiusSearch for many queries: the k nearest with d^2 < r^2:
+        (idx int32 (m,k), d2 float32 (m,k), count int32 (m,))."""
+        q = _as_host_points(queries)
+        m = q.shape[0]
+        idx = np.empty((m, k), np.int32)
+        d2 = np.empty((m, k), np.float32)
+        cnt = np.empty(m, np.int32)
+        check(lib().lvrhip_radius_search(self._h, q.ctypes.data_as(C.c_void_p), m, int(k), float(r),
+                                         idx.ctypes.data_as(C.c_void_p), d2.ctypes.data_as(C.c_void_p),
+                                         cnt.ctypes.data_as(C.c_void_p)))
+        return idx, d2, cnt
+
+    def getInterpolatedNormals(self, positions) -> np.ndarray:
+        """PointsetSurface::getInterpolatedNormal at each row of ``positions``: (m, 3) float32."""
+        q = _as_host_points(positions)
+        out = np.empty((q.shape[0], 3), np.float32)
+        check(lib().lvrhip_interpolated_normals(self._h, q.ctypes.data_as(C.c_void_p), q.shape[0],
+                                                out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def setSearchMode(self, mode) -> None:
+        """"auto" (best-first for searches of at most 32 neighbours), "tile" (always the tile kernel)
+        or "wave" (the wave-per-query best-first search for every query)."""
+        m = {"auto": 0, "tile": 1, "wave": 2}[mode] if isinstance(mode, str) else int(mode)
+        check(lib().lvrhip_set_search_mode(self._h, m))
+
     def distances(self, query_points, voxel_size: float, out=None):
         """CudaSurface::distances semantics: returns (proj, eucl, invalid) per query point.
 
@@ -292,12 +352,18 @@ class HipSurface:
         if _is_device_tensor(query_points):
             import torch
 
-            qt = query_points.contiguous()
+            qt = _device_points(query_points, "query_points")
             m = int(qt.shape[0])
             if out is None:
                 out = (torch.empty(m, dtype=torch.float32, device=qt.device),
                        torch.empty(m, dtype=torch.float32, device=qt.device),
                        torch.empty(m, dtype=torch.uint8, device=qt.device))
+            else:
+                if len(out) != 3:
+                    raise ValueError("out must be a tuple (proj, eucl, invalid)")
+                _check_out(out[0], (m,), torch.float32, qt.device, "out[0] (proj)")
+                _check_out(out[1], (m,), torch.float32, qt.device, "out[1] (eucl)")
+                _check_out(out[2], (m,), torch.uint8, qt.device, "out[2] (invalid)")
             check(lib().lvrhip_distances_device(self._h, C.c_void_p(qt.data_ptr()), m, float(voxel_size),
                                                 C.c_void_p(out[0].data_ptr()), C.c_void_p(out[1].data_ptr()),
                                                 C.c_void_p(out[2].data_ptr())))

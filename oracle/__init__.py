@@ -38,6 +38,8 @@ def _l():
         l.orc_surface_bbox.argtypes = [vp, vp]
         l.orc_knn.argtypes = [vp, vp, i64, i32, vp, vp]
         l.orc_knn_brute.argtypes = [vp, i64, vp, i64, i32, vp, vp]
+        l.orc_radius.argtypes = [vp, vp, i64, i32, f32, vp, vp, vp]
+        l.orc_interpolated_normal.argtypes = [vp, vp, i32, vp, i64, vp]
         l.orc_normals.argtypes = [vp, i32, i32, vp, vp, i64, vp, vp, vp, vp]
         l.orc_interpolate.argtypes = [vp, vp, i32, vp, i64, vp]
         l.orc_distance.argtypes = [vp, vp, i32, vp, i64, vp, vp]
@@ -110,6 +112,23 @@ class Surface:
         m = self.n if sub is None else sub.shape[0]
         out = np.empty((m, 3), np.float32)
         _l().orc_interpolate(self._s, _p(nin), ki, _p(sub), 0 if sub is None else m, _p(out))
+        return out
+
+    def radius(self, q, k: int, r: float):
+        """SearchKDTree::radiusSearch: (idx (m,k), d2 (m,k), count (m,))."""
+        q = _f32(q).reshape(-1, 3)
+        idx = np.empty((q.shape[0], k), np.int32)
+        d2 = np.empty((q.shape[0], k), np.float32)
+        cnt = np.empty(q.shape[0], np.int32)
+        _l().orc_radius(self._s, _p(q), q.shape[0], k, float(r), _p(idx), _p(d2), _p(cnt))
+        return idx, d2, cnt
+
+    def interpolated_normal(self, normals, ki: int, q):
+        """PointsetSurface::getInterpolatedNormal at the rows of q: (m, 3) float32."""
+        nin = _f32(normals).reshape(-1, 3)
+        q = _f32(q).reshape(-1, 3)
+        out = np.empty((q.shape[0], 3), np.float32)
+        _l().orc_interpolated_normal(self._s, _p(nin), ki, _p(q), q.shape[0], _p(out))
         return out
 
     def distance(self, normals, kd: int, q):

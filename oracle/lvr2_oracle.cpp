@@ -107,6 +107,21 @@ public:
         }
     }
 
+    // Contract radius search (SearchKDTree::radiusSearch, KDTree.tcc:368-386): the k smallest
+    // keys among the points with d2_f32 < r2.  The reference starts its worst distance at r^2
+    // (KDTree.tcc:101-110) and admits a point only below it (KDTree.tcc:236); here the heap
+    // admits keys below (r2, -) and the far side is pruned by the smaller of r2 and the worst.
+    void radius_contract(const float* q, int k, float r2, std::vector<KeyC>& out) const
+    {
+        std::priority_queue<KeyC> heap;
+        if (m_n > 0 && k > 0) search_cr(m_root, q, (size_t)k, r2, heap);
+        out.resize(heap.size());
+        for (size_t i = out.size(); i-- > 0;) {
+            out[i] = heap.top();
+            heap.pop();
+        }
+    }
+
     // Reference search: same traversal, KDTree.tcc:99-126 and 186-248.
     void knn_reference(const float* qf, int k, std::vector<KeyR>& out) const
     {
@@ -185,6 +200,32 @@ private:
         float dd = q[nd.axis] - s;
         float bound = dd * dd;
         if (heap.size() < k || !(bound > heap.top().d2)) search_c(second, q, k, heap);
+    }
+
+    void search_cr(int32_t id, const float* q, size_t k, float r2, std::priority_queue<KeyC>& heap) const
+    {
+        const Node& nd = m_nodes[id];
+        if (nd.lesser < 0) {
+            for (size_t i = nd.start; i < nd.end; i++) {
+                KeyC key{d2_f32(q, m_pts[i].p), m_pts[i].index};
+                if (!(key.d2 < r2)) continue;
+                if (heap.size() < k) {
+                    heap.push(key);
+                } else if (key < heap.top()) {
+                    heap.pop();
+                    heap.push(key);
+                }
+            }
+            return;
+        }
+        bool lfirst = (double)q[nd.axis] < nd.split;
+        int32_t first = lfirst ? nd.lesser : nd.greater;
+        int32_t second = lfirst ? nd.greater : nd.lesser;
+        search_cr(first, q, k, r2, heap);
+        float s = (float)nd.split;
+        float dd = q[nd.axis] - s;
+        float bound = dd * dd;
+        if (bound < r2 && (heap.size() < k || !(bound > heap.top().d2))) search_cr(second, q, k, r2, heap);
     }
 
     void search_r(int32_t id, const double* q, size_t k, std::priority_queue<KeyR>& heap, double& worst) const
@@ -589,6 +630,24 @@ int orc_knn(const orc_surface* s, const float* q, int64_t m, int k, int32_t* out
     return 0;
 }
 
+int orc_radius(const orc_surface* s, const float* q, int64_t m, int k, float r, int32_t* out_idx, float* out_d2,
+               int32_t* out_count)
+{
+    const float r2 = r * r;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < m; i++) {
+        std::vector<KeyC> keys;
+        s->tree->radius_contract(q + 3 * i, k, r2, keys);
+        const size_t got = keys.size();
+        for (size_t j = 0; j < (size_t)k; j++) {
+            out_idx[i * k + j] = j < got ? (int32_t)keys[j].idx : -1;
+            if (out_d2) out_d2[i * k + j] = j < got ? keys[j].d2 : INFINITY;
+        }
+        if (out_count) out_count[i] = (int32_t)got;
+    }
+    return 0;
+}
+
 int orc_knn_brute(const float* xyz, int64_t n, const float* q, int64_t m, int k, int32_t* out_idx, float* out_d2)
 {
 #pragma omp parallel for schedule(dynamic, 16)
@@ -681,6 +740,32 @@ int orc_interpolate(const orc_surface* s, const float* nin, int ki, const int64_
         out[3 * t] = r.x;
         out[3 * t + 1] = r.y;
         out[3 * t + 2] = r.z;
+    }
+    return 0;
+}
+
+int orc_interpolated_normal(const orc_surface* s, const float* normals, int ki, const float* q, int64_t m,
+                            float* out)
+{
+    // PointsetSurface::getInterpolatedNormal (PointsetSurface.tcc:61-74): result is a
+    // default Normal (0, 1, 0) (Normal.hpp:62); each neighbour's normal is read through
+    // Normal(x, y, z), which normalises (Normal.tcc:49-54); BaseVector::operator+= and /= add
+    // and divide component-wise; the returned copy is not normalised again.
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t i = 0; i < m; i++) {
+        std::vector<uint32_t> ids;
+        s->ksearch(q + 3 * i, ki, ids);
+        V3 r{0.0f, 1.0f, 0.0f};
+        for (uint32_t j : ids) {
+            const V3 n = normalized(V3{normals[3 * (size_t)j], normals[3 * (size_t)j + 1], normals[3 * (size_t)j + 2]});
+            r.x += n.x;
+            r.y += n.y;
+            r.z += n.z;
+        }
+        const float c = (float)ki;
+        out[3 * i] = r.x / c;
+        out[3 * i + 1] = r.y / c;
+        out[3 * i + 2] = r.z / c;
     }
     return 0;
 }

@@ -39,6 +39,17 @@ void orc_surface_bbox(const orc_surface* s, float* out6);
 int orc_knn(const orc_surface* s, const float* q, int64_t m, int k,
             int32_t* out_idx, float* out_d2);
 
+/* SearchKDTree::radiusSearch (KDTree.tcc:368-386) under the contract ordering: the k
+ * smallest keys among the points with fp32 d^2 < fp32 (r * r); rows padded with -1 / inf,
+ * out_count[i] = neighbours found (out_d2 / out_count may be NULL). */
+int orc_radius(const orc_surface* s, const float* q, int64_t m, int k, float r, int32_t* out_idx, float* out_d2,
+               int32_t* out_count);
+
+/* PointsetSurface::getInterpolatedNormal (PointsetSurface.tcc:61-74) at m positions:
+ * ((0,1,0) + sum of the ki nearest points' normalised normals) / ki.  out: m*3 floats. */
+int orc_interpolated_normal(const orc_surface* s, const float* normals, int ki, const float* q, int64_t m,
+                            float* out);
+
 /* O(N*M) brute force under the contract ordering; pins orc_knn on small inputs. */
 int orc_knn_brute(const float* xyz, int64_t n, const float* q, int64_t m, int k,
                   int32_t* out_idx, float* out_d2);

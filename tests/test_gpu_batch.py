@@ -211,3 +211,48 @@ def test_autotune_keeps_results(project):
     np.testing.assert_array_equal(s.normalsStats(), stats)
     # other launch shapes sum the moments in another order: fp32 rounding differences only
     assert np.quantile(axis_angles(before, s.getNormals()), 0.999) < 1e-5
+
+
+def test_outdoor_project_64_scans_full_size(oracle_mod):
+    """The batched configuration at its full size: 64 scans of the 200 x 200 m outdoor scene, each
+    ray-cast from its own scanner pose, sizes log-uniform in [50,000, 2,000,000] (about 30M
+    points), per-scan flip points, kn 50, ki 20, in one handle (the per-scan loop of
+    estimateProjectNormals, ScanProjectUtils.cpp:361-410).  On sampled points of 8 scans:
+    adaptive k exact and normals within the single-scan bars against the oracle run on that
+    scan alone; hipGraph replays bit-exact with the eager pass."""
+    import torch
+
+    from lvr2_amd import HipSurface, scenes
+
+    sp = scenes.scan_project_outdoor(64)
+    n = len(sp.points)
+    print(f"outdoor project: {len(sp.sizes)} scans, {n} points", flush=True)
+    assert 15_000_000 < n < 60_000_000
+    t = torch.from_numpy(sp.points).cuda()
+    s = HipSurface(t, scan_sizes=sp.sizes)
+    s.setKn(sp.kn)
+    s.setKi(0)
+    s.setScanFlippoints(sp.flips)
+    s.calculateNormals()
+    g = s.getNormals()
+    k_gpu = s.adaptiveK()
+    offs = np.concatenate([[0], np.cumsum(sp.sizes)])
+    order = np.argsort(sp.sizes)
+    picks = sorted(set([int(order[0]), int(order[-1])] + [int(x) for x in order[np.linspace(1, 62, 6).astype(int)]]))
+    rng = np.random.default_rng(9)
+    for sc in picks:
+        p = sp.points[offs[sc]: offs[sc + 1]]
+        o = oracle_mod.Surface(p)
+        sub = np.sort(rng.choice(len(p), 200, replace=False))
+        _, n64, k_or, gap = o.normals(sp.kn, 3, tuple(sp.flips[sc]), subset=sub, gap=True)
+        np.testing.assert_array_equal(k_gpu[offs[sc] + sub], k_or, err_msg=f"scan {sc}")
+        gs = g[offs[sc] + sub]
+        flipdir = sp.flips[sc].astype(np.float64) - p[sub].astype(np.float64)
+        amb = np.abs(np.sum(n64 * flipdir, 1)) < 1e-4 * np.linalg.norm(flipdir, axis=1)
+        good = (gap >= GAP_MIN) & ~amb
+        a = angles(gs, n64)
+        assert np.max(a[good]) <= NORMAL_TOL, (sc, np.max(a[good]))
+        near = (gap >= NEAR_GAP) & (gap < GAP_MIN) & ~amb
+        assert not near.any() or np.max(a[near]) <= NEAR_TOL, sc
+    s.setKi(sp.ki)
+    _graph_replays(s, n, replays=2)

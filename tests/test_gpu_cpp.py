@@ -1,0 +1,76 @@
+"""The C++ drop-in (include/lvr2_amd/HipSurface.hpp) run where lvr2 runs CudaSurface: the
+--useGPU block of lvr2_reconstruct (Main.cpp:356-373) and LargeScaleReconstruction's GPU block
+(LargeScaleReconstruction.tcc:640-688), replayed by tests/cpp/hipsurface_check (built by
+__graft_entry__.build() through tests/cpp/Makefile).
+
+Bars: the normals calculateNormals() returns are the interpolated normals of
+calculateSurfaceNormals (angle to the oracle's interpolation of its fp32 normals <= 1e-4 rad
+for >= 99.99 % of the points whose interpolated normal is not near-orthogonal to the flip
+direction); m_distance and m_invalid are bit-exact against the oracle given those normals."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, axis_angles
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "tests", "cpp", "hipsurface_check")
+
+
+def _exe():
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built (run __graft_entry__.build())")
+    return EXE
+
+
+def test_lvr2_call_sites_through_cpp_dropin(tmp_path, oracle_mod):
+    from lvr2_amd import scenes
+
+    w = scenes.plain_small()
+    o = oracle_mod.Surface(w.points)
+    v, cells, _ = oracle_mod.grid(w.points, o.bbox(), w.voxel, extrude=True)
+    w.points.tofile(tmp_path / "points.f32")
+    v.astype(np.float32).tofile(tmp_path / "queries.f32")
+    args = [_exe(), "run", str(tmp_path / "points.f32"), str(len(w.points)), str(w.kn), str(w.ki), str(w.kd),
+            *(repr(float(c)) for c in w.flip), repr(float(w.voxel)), str(tmp_path / "queries.f32"), str(len(v)),
+            str(tmp_path)]
+    print("running", " ".join(args[:2]), len(w.points), "points", len(v), "vertices", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    n_main = np.fromfile(tmp_path / "normals_main.f32", np.float32).reshape(-1, 3)
+    n_lsr = np.fromfile(tmp_path / "normals_lsr.f32", np.float32).reshape(-1, 3)
+    np.testing.assert_array_equal(n_main, n_lsr)
+    # calculateNormals = estimate + interpolate (CudaSurface::GPU_NN), against the oracle
+    n32, _, _ = o.normals(w.kn, 3, w.flip)
+    oi = o.interpolate(n32, w.ki)
+    flipdir = np.asarray(w.flip, np.float64) - w.points.astype(np.float64)
+    stable = np.abs(np.sum(oi * flipdir, 1)) > 1e-3 * np.linalg.norm(flipdir, axis=1)
+    a = axis_angles(n_main[stable], oi[stable])
+    print(f"interpolated normals vs oracle: p99.99 {np.quantile(a, 0.9999):.2e} max {a.max():.2e} rad")
+    assert np.quantile(a, 0.9999) <= 1e-4
+    # not the un-interpolated estimate: interpolation moves the normals far more than that error
+    assert np.median(axis_angles(n_main, n32)) > 100 * np.median(a)
+    # m_distance / m_invalid given those normals, bit-exact
+    dist = np.fromfile(tmp_path / "distance.f32", np.float32)
+    inv = np.fromfile(tmp_path / "invalid.u8", np.uint8)
+    oproj, oeucl = o.distance(n_lsr, w.kd, v)
+    np.testing.assert_array_equal(dist, oproj)
+    oinv, _ = oracle_mod.calc_distance_values(oeucl, w.voxel, cells)
+    np.testing.assert_array_equal(inv, oinv)
+
+
+def test_out_of_memory_contract(tmp_path):
+    """A refused device allocation (LVRHIP_FAIL_ALLOC_AFTER, refused on the host) surfaces as
+    LVRHIP_ENOMEM with "out of memory" in the C ABI and as the std::runtime_error that
+    LargeScaleReconstruction.tcc:656-662 catches to shrink its chunks; the handle then works."""
+    rng = np.random.default_rng(0)
+    p = rng.normal(size=(20000, 3)).astype(np.float32)
+    p.tofile(tmp_path / "points.f32")
+    env = {k: v for k, v in os.environ.items() if k != "LVRHIP_FAIL_ALLOC_AFTER"}
+    r = subprocess.run([_exe(), "oom", str(tmp_path / "points.f32"), str(len(p))], capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    assert "out of memory" in r.stdout

@@ -386,44 +386,53 @@ def test_lattice_bitexact_room_and_negative_coordinates(oracle_mod):
     np.testing.assert_array_equal(cd.cpu().numpy(), ocells)
 
 
-def test_large_cloud_32m(oracle_mod):
-    """The large-cloud configuration at its full size: 32,000,000 points (the room scan on an
-    8000 x 4000 ray grid), kn 100, ki 30, kd 10, voxel 0.1: LBVH build at scale, the adaptive
-    normals (k up to 3200), interpolation, lattice and SDF; exact kNN and adaptive k on samples."""
+def test_large_outdoor_32m(oracle_mod):
+    """The large single cloud at its full size: 32,000,000 points ray-cast from the centre of the
+    200 x 200 x 30 m outdoor scene (Perlin-noise terrain, 500 box buildings), 3 mm noise, kn 100,
+    ki 30, kd 10, voxel 0.1: LBVH build at scale, the adaptive normals (k up to 3200),
+    interpolation, lattice and SDF.  Exact kNN and adaptive k on samples; SDF bit-exact against
+    the oracle given the library's normals on sampled lattice vertices."""
     import torch
 
     from lvr2_amd import HipSurface, scenes
 
-    w = scenes.room_scan(2, 8000, 4000, 40, 0.003)
-    kn, ki, kd, voxel = 100, 30, 10, 0.1
-    t = torch.from_numpy(w).cuda()
+    w = scenes.large_outdoor()
+    print(f"outdoor cloud: {len(w.points)} points, scanner {w.flip}", flush=True)
+    t = torch.from_numpy(w.points).cuda()
     s = HipSurface(t)
-    s.setKn(kn)
-    s.setKi(ki)
-    s.setKd(kd)
-    s.setFlippoint(0, 0, 0)
+    s.setKn(w.kn)
+    s.setKi(w.ki)
+    s.setKd(w.kd)
+    s.setFlippoint(*w.flip)
     s.calculateNormals()
     k_gpu = s.adaptiveK()
     s.interpolateNormals()
     n = torch.empty_like(t)
     s.getNormals(n)
-    v, cells, _ = s.grid(voxel, True, device=True)
-    proj, eucl, inv = s.distances(v, voxel)
+    v, cells, _ = s.grid(w.voxel, True, device=True)
+    proj, eucl, inv = s.distances(v, w.voxel)
     torch.cuda.synchronize()
-    nn = n.cpu().numpy().astype(np.float64)
-    np.testing.assert_allclose(np.linalg.norm(nn, axis=1), 1.0, atol=1e-5)
-    assert np.mean(np.sum(nn * -w.astype(np.float64), 1) >= 0) > 0.999
+    nn = n.cpu().numpy()
+    np.testing.assert_allclose(np.linalg.norm(nn.astype(np.float64), axis=1), 1.0, atol=1e-5)
+    flipdir = np.asarray(w.flip, np.float64) - w.points.astype(np.float64)
+    assert np.mean(np.sum(nn * flipdir, 1) >= 0) > 0.999
     assert np.isfinite(proj.cpu().numpy()).all()
-    o = oracle_mod.Surface(w)
+    o = oracle_mod.Surface(w.points)
     rng = np.random.default_rng(5)
-    q = w[rng.integers(0, len(w), 1000)]
+    q = w.points[rng.integers(0, len(w.points), 1000)]
     gi, gd = s.kSearch(q, 100)
     oi, od = o.knn(q, 100)
     np.testing.assert_array_equal(gi, oi)
     np.testing.assert_array_equal(gd, od)
-    sub = np.sort(rng.choice(len(w), 300, replace=False))
-    _, _, k_or = o.normals(kn, 3, (0.0, 0.0, 0.0), subset=sub, fp64=True)
+    sub = np.sort(rng.choice(len(w.points), 300, replace=False))
+    _, _, k_or = o.normals(w.kn, 3, w.flip, subset=sub, fp64=True)
     np.testing.assert_array_equal(k_gpu[sub], k_or)
+    vs = rng.choice(v.shape[0], 2000, replace=False)
+    vh = v[torch.from_numpy(vs).cuda()].cpu().numpy()
+    oproj, oeucl = o.distance(nn, w.kd, vh)
+    np.testing.assert_array_equal(proj.cpu().numpy()[vs], oproj)
+    np.testing.assert_array_equal(eucl.cpu().numpy()[vs], oeucl)
+    np.testing.assert_array_equal(inv.cpu().numpy()[vs], (oeucl.astype(np.float64) > 1.75 * w.voxel).astype(np.uint8))
 
 
 @pytest.mark.parametrize("n", [1, 3, 17, 40, 300])

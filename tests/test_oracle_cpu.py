@@ -200,3 +200,45 @@ def test_calc_distance_values_rule(oracle_mod):
     inv, keep = oracle_mod.calc_distance_values(eucl, 1.0, cells)
     np.testing.assert_array_equal(inv, [0, 0, 0, 1, 1])
     np.testing.assert_array_equal(keep, [1, 0])
+
+
+def _numpy_keys(p, q):
+    """fp32 d^2 of the contract, (dx*dx + dy*dy) + dz*dz, for every (query, point) pair."""
+    d = q[:, None, :].astype(np.float32) - p[None, :, :].astype(np.float32)
+    sq = d * d
+    return (sq[..., 0] + sq[..., 1]) + sq[..., 2]
+
+
+@pytest.mark.parametrize("n,k,r", [(1, 3, 1.0), (40, 5, 0.5), (2000, 16, 0.3), (2000, 64, 0.3), (500, 8, 0.0)])
+def test_radius_search_matches_numpy(oracle_mod, n, k, r):
+    # SearchKDTree::radiusSearch: the k nearest among the points strictly inside r, by a
+    # search that starts its bound at r^2; pinned by a numpy filter-and-sort over all points
+    rng = np.random.default_rng(n + k)
+    p = np.round(rng.normal(size=(n, 3)) * 4).astype(np.float32) / 8  # ties, points exactly at r
+    q = np.concatenate([p[: min(n, 30)], rng.normal(size=(20, 3)).astype(np.float32)])
+    idx, d2, cnt = oracle_mod.Surface(p).radius(q, k, r)
+    D = _numpy_keys(p, q)
+    r2 = np.float32(r) * np.float32(r)
+    for i in range(q.shape[0]):
+        inside = np.flatnonzero(D[i] < r2)
+        order = inside[np.lexsort((inside, D[i, inside]))][:k]
+        assert cnt[i] == len(order)
+        np.testing.assert_array_equal(idx[i, : len(order)], order)
+        np.testing.assert_array_equal(d2[i, : len(order)], D[i, order])
+        assert (idx[i, len(order):] == -1).all() and np.isinf(d2[i, len(order):]).all()
+
+
+def test_interpolated_normal_hand_case(oracle_mod):
+    # PointsetSurface::getInterpolatedNormal: ((0,1,0) + sum of the normalised normals of the
+    # ki nearest) / ki, not normalised again.  Three points on the x axis, query near point 0.
+    p = np.array([[0, 0, 0], [1, 0, 0], [3, 0, 0]], np.float32)
+    nrm = np.array([[0, 0, 2], [0, 3, 0], [5, 0, 0]], np.float32)  # lengths 2, 3, 5
+    s = oracle_mod.Surface(p)
+    out = s.interpolated_normal(nrm, 2, np.array([[0.1, 0, 0]], np.float32))
+    np.testing.assert_allclose(out[0], [0.0, (1 + 1) / 2, 1 / 2], rtol=0, atol=1e-7)
+    out = s.interpolated_normal(nrm, 3, np.array([[2.9, 0, 0]], np.float32))
+    np.testing.assert_allclose(out[0], [1 / 3, 2 / 3, 1 / 3], rtol=0, atol=1e-7)
+    # a zero normal stays zero (BaseVector::normalize skips the null vector)
+    nrm[1] = 0
+    out = s.interpolated_normal(nrm, 2, np.array([[0.9, 0, 0]], np.float32))
+    np.testing.assert_allclose(out[0], [0.0, 1 / 2, 1 / 2], rtol=0, atol=1e-7)
