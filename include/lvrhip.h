@@ -188,6 +188,21 @@ int lvrhip_distances(lvrhip_t h, const float* q, size_t m, float voxel, float* o
 int lvrhip_distances_device(lvrhip_t h, const float* d_q, size_t m, float voxel, float* d_proj, float* d_eucl,
                             uint8_t* d_invalid);
 
+/* lvrhip_distances_device without any allocation: no counterpart in the reference; for callers
+ * that cannot allocate (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as lvrhip_distances_device (bit-identical), for at
+ * most LVRHIP_NOALLOC_MAX_QUERIES queries per call and kd <= 32 (LVRHIP_EINVAL otherwise).  Its
+ * scratch is one static device array of LVRHIP_NOALLOC_SCRATCH_BYTES that every device holds
+ * for the life of the process, shared by all handles on that device: calls that overlap on one
+ * device (other streams, other threads) race and must be serialised by the caller.  Queries are
+ * searched in the caller's order (spatially coherent orders, such as the lattice of
+ * lvrhip_grid_build, run fastest).  Synchronises the handle's stream; a query whose search
+ * outgrows the stacks (never seen in practice) is reported as LVRHIP_ECAPACITY. */
+#define LVRHIP_NOALLOC_MAX_QUERIES 1048576
+#define LVRHIP_NOALLOC_SCRATCH_BYTES 25167872
+int lvrhip_distances_device_noalloc(lvrhip_t h, const float* d_q, size_t m, float voxel, float* d_proj,
+                                    float* d_eucl, uint8_t* d_invalid);
+
 /* Differentiation form (no lvr2 counterpart; the reverse mode of AdaptiveKSearchSurface::distance,
  * tcc:414-457, through calcPlaneIPCAExact, tcc:609-673): for the m positions d_q (device) and
  * upstream weights d_grad_proj[v] = dL/dproj_v (device), adds dL/dpoints to d_grad_xyz (n*3

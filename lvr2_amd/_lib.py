@@ -59,6 +59,8 @@ _SIGS = {
                                        C.c_void_p, C.c_void_p]),
     "lvrhip_interpolated_normals": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "lvrhip_set_search_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "lvrhip_distances_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_float, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p]),
 }
 
 STAT_GROUPS = ("round1", "round2", "round3", "round4", "round5", "merged_rounds", "sdf", "other")

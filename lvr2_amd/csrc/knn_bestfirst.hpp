@@ -38,9 +38,10 @@ constexpr int BF_SPILL = 48;    // further entries per lane in global memory (de
 // two parts are never addressed through one generic pointer), the rest in global memory.
 typedef __attribute__((address_space(3))) uint2 lds_uint2;
 struct BfLdsStack {
-    static constexpr int CAP = BF_STACK + BF_SPILL;
     lds_uint2* s;  // this lane's LDS entries, interleaved with the wave's other lanes
-    uint2* spill;  // this lane's BF_SPILL entries, interleaved likewise
+    uint2* spill;  // this lane's spill entries, interleaved likewise
+    int cap;       // BF_STACK + the spill entries per lane
+    __device__ __forceinline__ int capacity() const { return cap; }
     __device__ __forceinline__ uint2 get(int i) const
     {
         uint2 v;
@@ -245,7 +246,7 @@ __device__ __forceinline__ bool bf_search(const BvhView& bv, const Stack& stk, f
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (k + 1 < nv) {
-                    if (sp == Stack::CAP) return false;
+                    if (sp == stk.capacity()) return false;
                     stk.put(sp, make_uint2((uint32_t)c[k], fbits(d[k])));
                     sp++;
                 } else if (k + 1 == nv) {
@@ -303,15 +304,19 @@ __device__ __forceinline__ void bf_stats(unsigned long long* stats, bool valid, 
 // blocks of QUEUE_CHUNK consecutive chunks to the XCDs in turn, each XCD taking from its own
 // counter and then helping the others, so neighbouring chunks share an L2 and the costly regions
 // of a cloud are spread over all XCDs.  Without a queue, wave b takes chunk b.
+//
+// Query i is (qx[i qs], qy[i qs], qz[i qs]): qs = 1 for SoA coordinates, 3 for an AoS buffer
+// (qy = qx + 1, qz = qx + 2).  spill: spill_cap entries per lane for each wave of the grid.
 template <int K, class Out>
 __global__ __launch_bounds__(BF_THREADS) void bf_knn_kernel(const BvhView bv, const float* __restrict__ qx,
                                                             const float* __restrict__ qy,
-                                                            const float* __restrict__ qz, int64_t m, int t,
-                                                            uint64_t empty, Out out, unsigned long long* stats,
-                                                            uint32_t* queue, uint2* spill)
+                                                            const float* __restrict__ qz, int qs, int64_t m,
+                                                            int t, uint64_t empty, Out out,
+                                                            unsigned long long* stats, uint32_t* queue, uint2* spill,
+                                                            int spill_cap)
 {
     __shared__ uint2 stk[BF_STACK][BF_THREADS];
-    uint2* const my_spill = spill + (size_t)blockIdx.x * BF_SPILL * BF_THREADS + threadIdx.x;
+    uint2* const my_spill = spill + (size_t)blockIdx.x * spill_cap * BF_THREADS + threadIdx.x;
     const int lane = threadIdx.x;
     const int64_t nchunks = (m + BF_THREADS - 1) / BF_THREADS;
     const int xcd = (int)(blockIdx.x % QUEUE_SEGS);
@@ -342,10 +347,11 @@ __global__ __launch_bounds__(BF_THREADS) void bf_knn_kernel(const BvhView bv, co
         int leaves = 0, nodes = 0;
         bool ok = true;
         if (i < m) {  // no barrier in the search: lanes are independent
-            const float x = qx[i], y = qy[i], z = qz[i];
+            const float x = qx[i * qs], y = qy[i * qs], z = qz[i * qs];
             BfList<K> L;
             L.init(t, empty);
-            ok = bf_search<K>(bv, BfLdsStack{(lds_uint2*)&stk[0][lane], my_spill}, x, y, z, L, leaves, nodes);
+            ok = bf_search<K>(bv, BfLdsStack{(lds_uint2*)&stk[0][lane], my_spill, BF_STACK + spill_cap}, x, y, z, L,
+                              leaves, nodes);
             out.template emit<K>(i, L, t, ok, x, y, z);
             ok = ok && L.full();
         }
@@ -377,7 +383,7 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
 template <int K, class Out>
 __global__ __launch_bounds__(BF_THREADS) void bf_knn_wave_kernel(const BvhView bv, const float* __restrict__ qx,
                                                                  const float* __restrict__ qy,
-                                                                 const float* __restrict__ qz,
+                                                                 const float* __restrict__ qz, int qs,
                                                                  const uint32_t* __restrict__ list,
                                                                  const uint32_t* __restrict__ count, int t,
                                                                  uint64_t empty, Out out, unsigned long long* stats)
@@ -391,7 +397,7 @@ __global__ __launch_bounds__(BF_THREADS) void bf_knn_wave_kernel(const BvhView b
     for (int64_t j = blockIdx.x; j < n; j += gridDim.x) {
         const unsigned long long c0 = stats ? cycles() : 0ull;
         const int64_t i = list[j];
-        const float x = qx[i], y = qy[i], z = qz[i];
+        const float x = qx[i * qs], y = qy[i * qs], z = qz[i * qs];
         // lane j < t holds the j-th smallest key so far (empty: none)
         uint64_t bk = empty;
         uint32_t bs = ~0u;
@@ -572,7 +578,7 @@ struct BfSdfOut {
     const float* nx;
     const float* ny;
     const float* nz;
-    const uint32_t* qorig;  // query position -> caller's index
+    const uint32_t* qorig;  // query position -> caller's index (NULL: the same)
     float voxel;
     float* out_proj;
     float* out_eucl;
@@ -612,7 +618,8 @@ struct BfSdfOut {
                 bz += v[j][5];
             }
         }
-        sdf_emit(ax, ay, az, bx, by, bz, t, qx, qy, qz, voxel, qorig[i], out_proj, out_eucl, out_invalid);
+        sdf_emit(ax, ay, az, bx, by, bz, t, qx, qy, qz, voxel, qorig ? qorig[i] : (uint32_t)i, out_proj, out_eucl,
+                 out_invalid);
     }
 };
 

@@ -396,10 +396,12 @@ __global__ void mean_normal_rows_kernel(int64_t m, int k, int t, const int32_t* 
 
 // Best-first kNN launch for t <= BF_KMAX: the list length is the smallest of 8, 10, 16, 32 that
 // holds t (10: the signed distance's usual kd).
+// Query coordinates: SoA (qs = 1) or AoS (qs = 3, qy = qx + 1, qz = qx + 2).  With a queue the
+// grid is min(chunks, resident) waves; spill holds spill_cap entries per lane for each of them.
 template <class Out>
-void launch_bf(const BvhView& bv, const float* qx, const float* qy, const float* qz, int64_t m, int t, uint64_t empty,
-               const Out& out, hipStream_t st, unsigned long long* stats, uint32_t* queue, int64_t resident,
-               uint2* spill)
+void launch_bf(const BvhView& bv, const float* qx, const float* qy, const float* qz, int qs, int64_t m, int t,
+               uint64_t empty, const Out& out, hipStream_t st, unsigned long long* stats, uint32_t* queue,
+               int64_t resident, uint2* spill, int spill_cap)
 {
     if (m == 0) return;
     int64_t grid = ceil_div(m, BF_THREADS);
@@ -410,31 +412,38 @@ void launch_bf(const BvhView& bv, const float* qx, const float* qy, const float*
     }
     const unsigned g = (unsigned)grid;
     if (t <= 8)
-        bf_knn_kernel<8, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, m, t, empty, out, stats, queue, spill);
+        bf_knn_kernel<8, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, qs, m, t, empty, out, stats, queue, spill,
+                                                        spill_cap);
     else if (t <= 10)
-        bf_knn_kernel<10, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, m, t, empty, out, stats, queue, spill);
+        bf_knn_kernel<10, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, qs, m, t, empty, out, stats, queue, spill,
+                                                         spill_cap);
     else if (t <= 16)
-        bf_knn_kernel<16, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, m, t, empty, out, stats, queue, spill);
+        bf_knn_kernel<16, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, qs, m, t, empty, out, stats, queue, spill,
+                                                         spill_cap);
     else
-        bf_knn_kernel<32, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, m, t, empty, out, stats, queue, spill);
+        bf_knn_kernel<32, Out><<<g, BF_THREADS, 0, st>>>(bv, qx, qy, qz, qs, m, t, empty, out, stats, queue, spill,
+                                                         spill_cap);
     LVRHIP_LAUNCH_CHECK();
 }
 
 // The wave-per-query pass over list[0, *count) (the queries the lane pass left).
 template <class Out>
-void launch_bf_wave(const BvhView& bv, const float* qx, const float* qy, const float* qz, const uint32_t* list,
-                    const uint32_t* count, int t, uint64_t empty, const Out& out, hipStream_t st,
-                    unsigned long long* stats)
+void launch_bf_wave(const BvhView& bv, const float* qx, const float* qy, const float* qz, int qs,
+                    const uint32_t* list, const uint32_t* count, int t, uint64_t empty, const Out& out,
+                    hipStream_t st, unsigned long long* stats)
 {
     const unsigned grid = BF_WAVE_BLOCKS;
     if (t <= 8)
-        bf_knn_wave_kernel<8, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, list, count, t, empty, out, stats);
+        bf_knn_wave_kernel<8, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, qs, list, count, t, empty, out, stats);
     else if (t <= 10)
-        bf_knn_wave_kernel<10, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, list, count, t, empty, out, stats);
+        bf_knn_wave_kernel<10, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, qs, list, count, t, empty, out,
+                                                                 stats);
     else if (t <= 16)
-        bf_knn_wave_kernel<16, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, list, count, t, empty, out, stats);
+        bf_knn_wave_kernel<16, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, qs, list, count, t, empty, out,
+                                                                 stats);
     else
-        bf_knn_wave_kernel<32, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, list, count, t, empty, out, stats);
+        bf_knn_wave_kernel<32, Out><<<grid, BF_THREADS, 0, st>>>(bv, qx, qy, qz, qs, list, count, t, empty, out,
+                                                                 stats);
     LVRHIP_LAUNCH_CHECK();
 }
 
@@ -534,6 +543,47 @@ void launch_tile_p(const BvhView& bv, const QueryView& qv, const TileParams& prm
     LVRHIP_LAUNCH_CHECK();
 }
 
+// ---- the no-allocation signed distances (lvrhip_distances_device_noalloc) -----------------------
+// Its scratch lives in one file-scope device array, `scratch`, that every device holds from the
+// moment the library's code object is loaded until the process ends.  Layout (byte offsets):
+// redo flags and redo list (NA_MAX_QUERIES entries each), compaction block sums, two counters,
+// the work queue, and the lane search's stack spill (NA_SPILL entries per lane for NA_WAVES
+// waves).  Host code never uses the array's name: it asks hipGetSymbolAddress for the address
+// on each device once (na_scratch) and passes that pointer to the kernels.
+constexpr int64_t NA_MAX_QUERIES = 1 << 20;
+constexpr int NA_SPILL = 16;
+constexpr int64_t NA_WAVES = 2048;
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NA_FLAGS = 0;
+constexpr size_t NA_LIST = na_align(NA_FLAGS + 4 * (size_t)NA_MAX_QUERIES);
+constexpr size_t NA_BSUM = na_align(NA_LIST + 4 * (size_t)NA_MAX_QUERIES);
+constexpr size_t NA_COUNT = na_align(NA_BSUM + 4 * (size_t)(NA_MAX_QUERIES / CF_TILE + 2));
+constexpr size_t NA_QUEUE = na_align(NA_COUNT + 4 * 4);
+constexpr size_t NA_SPILL_OFF = na_align(NA_QUEUE + 4 * (size_t)(QUEUE_SEGS * QUEUE_STRIDE));
+constexpr size_t NA_BYTES = na_align(NA_SPILL_OFF + sizeof(uint2) * (size_t)NA_SPILL * BF_THREADS * NA_WAVES);
+static_assert(NA_BYTES == LVRHIP_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
+
+static __device__ __align__(256) uint4 scratch[NA_BYTES / sizeof(uint4)];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mu;
+char* g_na_ptr[NA_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device (looked up once per device).
+char* na_scratch(int device)
+{
+    if (device < 0 || device >= NA_MAX_DEVICES) throw Error(LVRHIP_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(LVRHIP_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_na_ptr[device] = static_cast<char*>(p);
+    }
+    return g_na_ptr[device];
+}
+
 }  // namespace
 
 struct lvrhip_surface {
@@ -626,8 +676,11 @@ struct lvrhip_surface {
     hipGraphExec_t gexec = nullptr;
     hipStream_t graph_stream = nullptr;
 
+    uint32_t* h_pin = nullptr;  // pinned host words for readbacks that must not stage (hipHostMalloc)
+
     ~lvrhip_surface()
     {
+        if (h_pin) (void)hipHostFree(h_pin);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (own_gst && gst) (void)hipStreamDestroy(gst);
@@ -761,6 +814,7 @@ struct lvrhip_surface {
         d_cnt.reserve(CNT_SLOTS);
         fill_u32_kernel<<<1, 64, 0, st>>>(d_cnt.p, CNT_SLOTS, 0u);
         LVRHIP_LAUNCH_CHECK();
+        if (!h_pin) LVRHIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_pin), 64, hipHostMallocDefault));
         if (n == 0) return;
         d_off.reserve(ns + 1);
         d_pstart.reserve(ns + 1);
@@ -875,13 +929,19 @@ struct lvrhip_surface {
     void compact_flags(const uint32_t* flags, int64_t nq, const uint32_t* nq_dev, const uint32_t* qsel, uint32_t* out,
                        uint32_t* count_dev)
     {
+        s_bsum.reserve(std::max<int64_t>(ceil_div(nq, CF_TILE), 1) + 1);
+        compact_flags_with(flags, nq, nq_dev, qsel, out, count_dev, s_bsum.p);
+    }
+    // the same with the caller's block-sum buffer (ceil(nq / CF_TILE) + 1 entries)
+    void compact_flags_with(const uint32_t* flags, int64_t nq, const uint32_t* nq_dev, const uint32_t* qsel,
+                            uint32_t* out, uint32_t* count_dev, uint32_t* bsum)
+    {
         const int64_t nb = std::max<int64_t>(ceil_div(nq, CF_TILE), 1);
-        s_bsum.reserve(nb + 1);
-        flag_block_sums_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, nq_dev, s_bsum.p);
+        flag_block_sums_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, nq_dev, bsum);
         LVRHIP_LAUNCH_CHECK();
-        scan_exclusive_kernel<<<1, 1024, 0, st>>>(s_bsum.p, nb, count_dev);
+        scan_exclusive_kernel<<<1, 1024, 0, st>>>(bsum, nb, count_dev);
         LVRHIP_LAUNCH_CHECK();
-        compact_scatter_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, nq_dev, s_bsum.p, qsel, out);
+        compact_scatter_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(flags, nq, nq_dev, bsum, qsel, out);
         LVRHIP_LAUNCH_CHECK();
     }
 
@@ -1422,6 +1482,43 @@ struct lvrhip_surface {
         prof.collect();
     }
 
+    // distances_device without allocating: the queries in the caller's order (no Morton sort), the
+    // best-first searches with their scratch in the static array at scr, and no tile kernel (a
+    // query that outgrows the wave search's stack is reported as LVRHIP_ECAPACITY).
+    void distances_noalloc(const float* d_q, int64_t m, float voxel, float* d_proj, float* d_eucl, uint8_t* d_inv,
+                           char* scr)
+    {
+        if (!has_normals) throw Error(LVRHIP_ESTATE, "distances called before calculate_normals");
+        require_one_scan("distances");
+        if (m == 0) return;
+        if (n == 0) throw Error(LVRHIP_ESTATE, "distances on an empty point cloud");
+        if (m > NA_MAX_QUERIES)
+            throw Error(LVRHIP_EINVAL, "the no-allocation distances take at most " + std::to_string(NA_MAX_QUERIES) +
+                                           " queries per call (the static scratch size)");
+        const int t0 = clamp_k(kd);
+        if (t0 > BF_KMAX)
+            throw Error(LVRHIP_EINVAL, "the no-allocation distances take kd <= " + std::to_string(BF_KMAX));
+        uint32_t* flags = reinterpret_cast<uint32_t*>(scr + NA_FLAGS);
+        uint32_t* list = reinterpret_cast<uint32_t*>(scr + NA_LIST);
+        uint32_t* bsum = reinterpret_cast<uint32_t*>(scr + NA_BSUM);
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(scr + NA_COUNT);
+        uint32_t* queue = reinterpret_cast<uint32_t*>(scr + NA_QUEUE);
+        uint2* spill = reinterpret_cast<uint2*>(scr + NA_SPILL_OFF);
+        BfSdfOut o{px.p, py.p, pz.p, nx.p, ny.p, nz.p, nullptr, voxel, d_proj, d_eucl, d_inv, flags};
+        // every flag is written by the lane search; the queue is cleared by launch_bf
+        launch_bf(view(), d_q, d_q + 1, d_q + 2, 3, m, t0, BF_EMPTY, o, st, nullptr, queue,
+                  std::min(bf_resident, NA_WAVES), spill, NA_SPILL);
+        compact_flags_with(flags, m, nullptr, nullptr, list, cnt, bsum);
+        launch_bf_wave(view(), d_q, d_q + 1, d_q + 2, 3, list, cnt, t0, BF_EMPTY, o, st, nullptr);
+        compact_flags_with(flags, m, nullptr, nullptr, list, cnt + 1, bsum);
+        LVRHIP_CHECK(hipMemcpyAsync(h_pin, cnt + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        LVRHIP_CHECK(hipStreamSynchronize(st));
+        const uint32_t left = h_pin[0];
+        if (left != 0)
+            throw Error(LVRHIP_ECAPACITY, std::to_string(left) + " queries outgrew the search stacks of the "
+                                              "no-allocation distances; use lvrhip_distances_device");
+    }
+
     // ---- the PointsetGrid lattice (lattice.hpp) -------------------------------------------------
     DBuf<uint64_t> g_k0, g_k1, g_cells, g_cells2, g_first;
     DBuf<uint32_t> g_v0, g_v1, g_flags, g_idx, g_gid, g_vidx, g_vid, g_hist;
@@ -1668,11 +1765,11 @@ struct lvrhip_surface {
             // one spill area per wave of the grid
             const int64_t waves = q ? std::min(ceil_div(m, BF_THREADS), bf_resident) : ceil_div(m, BF_THREADS);
             s_spill.reserve((size_t)waves * BF_SPILL * BF_THREADS);
-            launch_bf(view(), s_qx.p, s_qy.p, s_qz.p, m, t0, empty, o, st, kstats_ptr(stats_group), q, bf_resident,
-                      s_spill.p);
+            launch_bf(view(), s_qx.p, s_qy.p, s_qz.p, 1, m, t0, empty, o, st, kstats_ptr(stats_group), q, bf_resident,
+                      s_spill.p, BF_SPILL);
         }
         compact_flags(s_redo.p, m, nullptr, nullptr, s_redo_list.p, counter(8));
-        launch_bf_wave(view(), s_qx.p, s_qy.p, s_qz.p, s_redo_list.p, counter(8), t0, empty, o, st,
+        launch_bf_wave(view(), s_qx.p, s_qy.p, s_qz.p, 1, s_redo_list.p, counter(8), t0, empty, o, st,
                        kstats_ptr(stats_group == 6 ? 5 : stats_group));
         compact_flags(s_redo.p, m, nullptr, nullptr, s_redo_list.p, counter(8));
     }
@@ -2121,6 +2218,19 @@ int lvrhip_distances_device(lvrhip_t h, const float* d_q, size_t m, float voxel,
         REQUIRE(m < (size_t)0xFFFFFFFFu, "too many queries");
         DeviceGuard dg(h->device);
         h->distances_device(d_q, (int64_t)m, voxel, d_proj, d_eucl, d_invalid);
+    });
+}
+
+int lvrhip_distances_device_noalloc(lvrhip_t h, const float* d_q, size_t m, float voxel, float* d_proj,
+                                    float* d_eucl, uint8_t* d_invalid)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        REQUIRE(m == 0 || d_q, "NULL query pointer");
+        REQUIRE(m <= (size_t)NA_MAX_QUERIES, "the no-allocation distances take at most " +
+                                                 std::to_string(NA_MAX_QUERIES) + " queries per call");
+        DeviceGuard dg(h->device);
+        h->distances_noalloc(d_q, (int64_t)m, voxel, d_proj, d_eucl, d_invalid, na_scratch(h->device));
     });
 }
 
