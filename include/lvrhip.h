@@ -127,8 +127,11 @@ int lvrhip_set_stream(lvrhip_t h, void* stream);
  * selected method, then the flip towards the flip point. */
 int lvrhip_calculate_normals(lvrhip_t h);
 
-/* Replaces CudaSurface::interpolateNormals (CudaSurface.cu:1255-1257, empty there) with
- * AdaptiveKSearchSurface::interpolateSurfaceNormals (tcc:293-335). No-op when ki == 0. */
+/* AdaptiveKSearchSurface::interpolateSurfaceNormals (tcc:293-335): each normal averaged with
+ * its ki nearest points' normals.  No-op when ki == 0.  The C++ drop-in runs it inside
+ * calculateNormals, where CudaSurface interpolates (CudaSurface.cu:1143-1148; its
+ * interpolateNormals, CudaSurface.cu:1255-1257, is empty).  ki up to 64 (clamped to the cloud's
+ * or the smallest scan's size); larger ki returns LVRHIP_EINVAL. */
 int lvrhip_interpolate_normals(lvrhip_t h);
 
 /* Use the given normals (n*3 floats, AoS, input order) instead of estimating them: the
@@ -227,12 +230,14 @@ int lvrhip_get_adaptive_k(lvrhip_t h, int32_t* out_k);
 
 /* The PointsetGrid lattice on the device (PointsetGrid.tcc:41-109 with HashGrid::addLatticePoints
  * and addBox, HashGrid.tcc:235-259, 333-378): cells floor(p / voxel) whose centre lies in the
- * cloud's bounding box, with extrude != 0 also their 26 neighbours (centre in the box), in
- * increasing (x, y, z) index order; each cell's 8 corners at centre + box_creation_table[v] *
- * voxel / 2 (FastReconstructionTables.hpp:86-95), a corner shared with an earlier cell reused, so
- * vertex ids follow first use as in the reference's sequential insertion.  Returns the vertex
- * and cell counts; the lattice stays in the handle (single-scan handles only).  Cell indices
- * must lie within +-2^20. */
+ * cloud's bounding box, with extrude != 0 also their 26 neighbours (centre in the box); each
+ * cell's 8 corners at centre + box_creation_table[v] * voxel / 2 (FastReconstructionTables.hpp:
+ * 86-95), a corner shared with an earlier cell reused, vertex ids in order of first use.  The
+ * reference inserts the cells in the iteration order of an unordered_set filled by OpenMP threads
+ * under a critical section (PointsetGrid.tcc:59-108), which varies between runs; here the cells
+ * are taken in increasing (x, y, z) index order, one of the reference's possible orders (same
+ * vertex and cell sets).  Returns the vertex and cell counts; the lattice stays in the handle
+ * (single-scan handles only).  Cell indices must lie within +-2^20. */
 int lvrhip_grid_build(lvrhip_t h, float voxel, int extrude, size_t* nv, size_t* nc);
 /* Copies of the last lattice: vertices (nv*3 floats, the positions lvrhip_distances takes),
  * cells (nc*8 vertex ids per cell, corner order of box_creation_table), cell index (nc*3 int32).

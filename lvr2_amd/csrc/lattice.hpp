@@ -4,9 +4,14 @@
 // floor(p / voxel) (HashGrid::calcIndex, HashGrid.hpp:263-281) whose centre (i + 0.5) voxel
 // (HashGrid.hpp:283-287) lies in the cloud's bounding box, optionally adds the 26 neighbours
 // of each (again only those whose centre is in the box), then HashGrid::addLatticePoints /
-// addBox (HashGrid.tcc:235-259, 333-378) visits the cells in index order and creates each
-// cell's 8 corners at centre + box_creation_table[v] * voxel / 2
-// (FastReconstructionTables.hpp:86-95), a corner shared with an earlier cell being reused.
+// addBox (HashGrid.tcc:235-259, 333-378) visits the cells and creates each cell's 8 corners at
+// centre + box_creation_table[v] * voxel / 2 (FastReconstructionTables.hpp:86-95), a corner
+// shared with an earlier cell being reused.  The reference visits the cells in the iteration
+// order of an std::unordered_set that its OpenMP threads fill under a critical section
+// (PointsetGrid.tcc:59-108), so its vertex numbering, and which cell a shared corner's fp32
+// position comes from, change from run to run.  This lattice fixes that order to increasing
+// (x, y, z) cell index, the order the CPU restatement uses: the vertex and cell sets equal the
+// reference's, the numbering and last-ulp corner positions match one of its possible orders.
 //
 // Here: cell keys are sorted and deduplicated with the radix sort; the 8 nc (corner, order)
 // pairs are sorted by corner so that equal corners are adjacent, each corner's first use

@@ -40,6 +40,8 @@ def _l():
         l.orc_knn_brute.argtypes = [vp, i64, vp, i64, i32, vp, vp]
         l.orc_radius.argtypes = [vp, vp, i64, i32, f32, vp, vp, vp]
         l.orc_interpolated_normal.argtypes = [vp, vp, i32, vp, i64, vp]
+        l.orc_interpolate_f64.argtypes = [vp, vp, i32, vp, i64, vp]
+        l.orc_distance_f64.argtypes = [vp, vp, i32, vp, i64, vp, vp]
         l.orc_normals.argtypes = [vp, i32, i32, vp, vp, i64, vp, vp, vp, vp]
         l.orc_interpolate.argtypes = [vp, vp, i32, vp, i64, vp]
         l.orc_distance.argtypes = [vp, vp, i32, vp, i64, vp, vp]
@@ -130,6 +132,24 @@ class Surface:
         out = np.empty((q.shape[0], 3), np.float32)
         _l().orc_interpolated_normal(self._s, _p(nin), ki, _p(q), q.shape[0], _p(out))
         return out
+
+    def interpolate_f64(self, normals64, ki: int, subset=None):
+        """interpolateSurfaceNormals in fp64 on the same neighbourhoods: (m, 3) float64."""
+        nin = np.ascontiguousarray(normals64, dtype=np.float64).reshape(-1, 3)
+        sub = None if subset is None else np.ascontiguousarray(subset, dtype=np.int64)
+        m = self.n if sub is None else sub.shape[0]
+        out = np.empty((m, 3), np.float64)
+        _l().orc_interpolate_f64(self._s, _p(nin), ki, _p(sub), 0 if sub is None else m, _p(out))
+        return out
+
+    def distance_f64(self, normals64, kd: int, q):
+        """AdaptiveKSearchSurface::distance in fp64: (proj, eucl) float64."""
+        nin = np.ascontiguousarray(normals64, dtype=np.float64).reshape(-1, 3)
+        q = _f32(q).reshape(-1, 3)
+        proj = np.empty(q.shape[0], np.float64)
+        eucl = np.empty(q.shape[0], np.float64)
+        _l().orc_distance_f64(self._s, _p(nin), kd, _p(q), q.shape[0], _p(proj), _p(eucl))
+        return proj, eucl
 
     def distance(self, normals, kd: int, q):
         nin = _f32(normals).reshape(-1, 3)

@@ -744,6 +744,63 @@ int orc_interpolate(const orc_surface* s, const float* nin, int ki, const int64_
     return 0;
 }
 
+int orc_interpolate_f64(const orc_surface* s, const double* nin, int ki, const int64_t* subset, int64_t nsub,
+                        double* out)
+{
+    const float* pts = s->pts.data();
+    const int64_t cnt = subset ? nsub : s->n;
+    // interpolateSurfaceNormals (tcc:312-327) on the same neighbourhoods, every sum in fp64
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t t = 0; t < cnt; t++) {
+        const int64_t i = subset ? subset[t] : t;
+        std::vector<uint32_t> ids;
+        s->ksearch(pts + 3 * i, ki, ids);
+        double m[3] = {nin[3 * i], nin[3 * i + 1], nin[3 * i + 2]};
+        for (uint32_t j : ids)
+            for (int c = 0; c < 3; c++) m[c] += nin[3 * (size_t)j + c];
+        const double l = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        for (int c = 0; c < 3; c++) out[3 * t + c] = l > 0 ? m[c] / l : m[c];
+    }
+    return 0;
+}
+
+int orc_distance_f64(const orc_surface* s, const double* normals, int kd, const float* q, int64_t m,
+                     double* out_proj, double* out_eucl)
+{
+    const float* pts = s->pts.data();
+    // AdaptiveKSearchSurface::distance (tcc:414-457) on the same neighbourhoods, in fp64
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t i = 0; i < m; i++) {
+        std::vector<uint32_t> ids;
+        s->ksearch(q + 3 * i, kd, ids);
+        if (ids.empty()) {
+            out_proj[i] = out_eucl[i] = FLT_MAX;
+            continue;
+        }
+        double c[3] = {0, 0, 0}, a[3] = {0, 0, 0};
+        for (uint32_t j : ids)
+            for (int k = 0; k < 3; k++) {
+                c[k] += pts[3 * (size_t)j + k];
+                a[k] += normals[3 * (size_t)j + k];
+            }
+        const double cnt = (double)ids.size();
+        for (int k = 0; k < 3; k++) {
+            c[k] /= cnt;
+            a[k] /= cnt;
+        }
+        const double l = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        double d[3], pr = 0, e = 0;
+        for (int k = 0; k < 3; k++) {
+            d[k] = (double)q[3 * i + k] - c[k];
+            pr += d[k] * (l > 0 ? a[k] / l : a[k]);
+            e += d[k] * d[k];
+        }
+        out_proj[i] = pr;
+        out_eucl[i] = std::sqrt(e);
+    }
+    return 0;
+}
+
 int orc_interpolated_normal(const orc_surface* s, const float* normals, int ki, const float* q, int64_t m,
                             float* out)
 {

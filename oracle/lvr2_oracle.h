@@ -70,6 +70,14 @@ int orc_normals(const orc_surface* s, int kn, int method, const float flip[3],
 int orc_interpolate(const orc_surface* s, const float* normals_in, int ki,
                     const int64_t* subset, int64_t nsub, float* out);
 
+/* The same interpolation in fp64 (fp64 input normals, fp64 sums and normalisation). */
+int orc_interpolate_f64(const orc_surface* s, const double* normals_in, int ki,
+                        const int64_t* subset, int64_t nsub, double* out);
+
+/* AdaptiveKSearchSurface::distance in fp64 on the same neighbourhoods (fp64 normals). */
+int orc_distance_f64(const orc_surface* s, const double* normals, int kd,
+                     const float* q, int64_t m, double* out_proj, double* out_eucl);
+
 /* AdaptiveKSearchSurface::distance (tcc:414-457) at m query positions. */
 int orc_distance(const orc_surface* s, const float* normals, int kd,
                  const float* q, int64_t m, float* out_proj, float* out_eucl);

@@ -209,8 +209,13 @@ def test_autotune_keeps_results(project):
     assert best == s.tuning()
     s.calculateNormals()
     np.testing.assert_array_equal(s.normalsStats(), stats)
-    # other launch shapes sum the moments in another order: fp32 rounding differences only
-    assert np.quantile(axis_angles(before, s.getNormals()), 0.999) < 1e-5
+    # other launch shapes sum the moments in another order: fp32 rounding differences only; where
+    # the FP64 eigen gap is >= GAP_MIN both are within NORMAL_TOL of FP64, so within twice that of
+    # each other
+    after = s.getNormals()
+    assert np.quantile(axis_angles(before, after), 0.999) < 1e-5
+    good = np.concatenate([ref[4] >= GAP_MIN if ref is not None else np.zeros(0, bool) for ref in per_scan])
+    assert np.max(axis_angles(before[good], after[good])) <= 2 * NORMAL_TOL
 
 
 def test_outdoor_project_64_scans_full_size(oracle_mod):

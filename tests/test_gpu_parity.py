@@ -114,8 +114,11 @@ def test_normals_pca_plain_small(c1):
     near = (gap >= NEAR_GAP) & (gap < GAP_MIN) & ~amb
     assert not near.any() or np.max(a[near]) <= NEAR_TOL
     assert np.max(axis_angles(g, n64)[gap >= GAP_MIN]) <= NORMAL_TOL
-    # the reference's own fp32 recursion is looser; report-only bound
-    assert np.quantile(axis_angles(g, n32)[good], 0.9999) < 1e-3
+    # against the fp32 restatement (the reference's own IPCA recursion): <= 1e-4 rad for 99.99 %
+    # of the points; the recursion itself is farther than 1e-4 from fp64 for 0.003 % of them
+    a32 = axis_angles(g, n32)
+    print(f"vs the fp32 restatement: p99.99 {np.quantile(a32, 0.9999):.2e}, max {a32.max():.2e} rad")
+    assert np.quantile(a32, 0.9999) <= 1e-4
 
 
 def test_normals_methods_ls_and_iterative(c1, oracle_mod):
@@ -201,7 +204,15 @@ def test_sdf_bitexact_on_grid_vertices(c1, oracle_mod):
 
 
 def test_end_to_end_plain_small(c1, oracle_mod):
-    w, o, n32 = c1[0], c1[1], c1[2]
+    """calculateNormals + interpolateNormals + distances on the plain scene's whole lattice against
+    the FP64 chain (FP64 plane fits and flips on the same neighbourhoods, FP64 interpolation,
+    FP64 distance).  Bar: |proj - proj64| <= 1e-5 voxel + 2 ulp(|proj64|) + 4 ulp(max |v|) (the
+    last term is the fp32 rounding of the reference's own formula at the vertex's coordinates:
+    its fp32 evaluation given exact normals is up to 3.2 ulp of |v| from fp64), and the sign of
+    proj (the marching-cubes index, FastBox.tcc:107-117) equal wherever |proj64| exceeds that
+    tolerance; excluded: vertices whose kd neighbours' interpolation neighbourhoods contain a
+    point with FP64 eigen gap < 1e-2 or a flip-ambiguous normal."""
+    w, o, n32, n64, k_or, gap = c1
     s = _surface(w.points)
     s.setKn(w.kn)
     s.setKi(w.ki)
@@ -210,17 +221,35 @@ def test_end_to_end_plain_small(c1, oracle_mod):
     s.calculateNormals()
     s.interpolateNormals()
     g = s.getNormals()
-    oi = o.interpolate(n32, w.ki)
-    flipdir = np.asarray(w.flip, np.float64) - w.points.astype(np.float64)
-    stable = np.abs(np.sum(oi * flipdir, 1)) > 1e-3 * np.linalg.norm(flipdir, axis=1)
-    v, cells, _ = oracle_mod.grid(w.points[::5], o.bbox(), w.voxel, extrude=False)
+    v, cells, _ = oracle_mod.grid(w.points, o.bbox(), w.voxel, extrude=True)
     proj, eucl, inv = s.distances(v, w.voxel)
-    oproj, oeucl = o.distance(oi, w.kd, v)
-    # same neighbourhoods; the mean normal differs by the normals' fp32 error only
-    np.testing.assert_allclose(eucl, oeucl, rtol=0, atol=1e-4)
-    assert np.median(np.abs(proj - oproj)) < 1e-4 * w.voxel
-    assert np.mean(np.sign(proj) == np.sign(oproj)) > 0.95
-    assert np.mean(axis_angles(g[stable], oi[stable]) < 1e-3) > 0.999
+    i64 = o.interpolate_f64(n64, w.ki)
+    p64, e64 = o.distance_f64(i64, w.kd, v)
+    flipdir = np.asarray(w.flip, np.float64) - w.points.astype(np.float64)
+    amb = np.abs(np.sum(n64 * flipdir, 1)) < 1e-4 * np.linalg.norm(flipdir, axis=1)
+    badp = (gap < GAP_MIN) | amb
+    ki_idx, _ = o.knn(w.points, w.ki)
+    bad_interp = badp | badp[ki_idx].any(axis=1)
+    kd_idx, _ = o.knn(v, w.kd)
+    good = ~bad_interp[kd_idx].any(axis=1)
+    vmax = np.max(np.abs(v), axis=1).astype(np.float32)
+    tol = 1e-5 * w.voxel + 2 * np.spacing(np.abs(p64).astype(np.float32)) + 4 * np.spacing(vmax)
+    err = np.abs(proj.astype(np.float64) - p64)
+    print(f"{len(v)} vertices, {np.mean(~good):.2%} excluded; |proj - proj64| / voxel: p50 "
+          f"{np.quantile(err[good], 0.5) / w.voxel:.2e} p99.9 {np.quantile(err[good], 0.999) / w.voxel:.2e} "
+          f"max {err[good].max() / w.voxel:.2e}; all vertices max {err.max() / w.voxel:.2e}")
+    assert np.all(err[good] <= tol[good]), np.max(err[good] / tol[good])
+    signed = good & (np.abs(p64) > tol)
+    assert np.array_equal(np.sign(proj[signed]), np.sign(p64[signed]))
+    # over all vertices the sign agrees wherever |proj64| clears the tolerance by 10x
+    clear = np.abs(p64) > 10 * tol
+    assert np.array_equal(np.sign(proj[clear]), np.sign(p64[clear]))
+    np.testing.assert_allclose(eucl, e64, rtol=1e-5, atol=4 * np.spacing(np.float32(np.abs(v).max())))
+    # the interpolated normals: <= 1e-4 rad from the FP32 chain for 99.99 % of the points whose
+    # interpolated normal is not near-orthogonal to the flip direction
+    oi = o.interpolate(n32, w.ki)
+    stable = np.abs(np.sum(oi * flipdir, 1)) > 1e-3 * np.linalg.norm(flipdir, axis=1)
+    assert np.quantile(axis_angles(g[stable], oi[stable]), 0.9999) <= 1e-4
 
 
 @pytest.mark.parametrize("merged", [True, False])
@@ -294,6 +323,7 @@ def test_room_4m_knn_and_orientation(oracle_mod):
     s = _surface(w.points)
     s.setKn(w.kn)
     s.setKi(w.ki)
+    s.setKd(w.kd)
     s.setFlippoint(*w.flip)
     s.calculateNormals()
     s.interpolateNormals()
@@ -303,9 +333,20 @@ def test_room_4m_knn_and_orientation(oracle_mod):
     rng = np.random.default_rng(4)
     q = w.points[rng.integers(0, len(w.points), 3000)]
     gi, gd = s.kSearch(q, 100)
-    oi, od = oracle_mod.Surface(w.points).knn(q, 100)
+    o = oracle_mod.Surface(w.points)
+    oi, od = o.knn(q, 100)
     np.testing.assert_array_equal(gi, oi)
     np.testing.assert_array_equal(gd, od)
+    # the signed distance at full size: 20,000 vertices of the 752,869-vertex lattice, proj,
+    # eucl and invalid bit-exact against the oracle given the library's normals
+    v, _, _ = s.grid(w.voxel, True)
+    assert len(v) == 752869
+    vs = v[rng.choice(len(v), 20000, replace=False)]
+    proj, eucl, inv = s.distances(vs, w.voxel)
+    oproj, oeucl = o.distance(g.astype(np.float32), w.kd, vs)
+    np.testing.assert_array_equal(proj, oproj)
+    np.testing.assert_array_equal(eucl, oeucl)
+    np.testing.assert_array_equal(inv, (oeucl.astype(np.float64) > 1.75 * w.voxel).astype(np.uint8))
 
 
 def test_repeatable_and_device_tensor_path(c1):
