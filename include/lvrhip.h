@@ -245,6 +245,37 @@ int lvrhip_grid_build(lvrhip_t h, float voxel, int extrude, size_t* nv, size_t* 
 int lvrhip_grid_get(lvrhip_t h, float* vertices, int32_t* cells, int32_t* cell_index);
 int lvrhip_grid_get_device(lvrhip_t h, float* d_vertices, int32_t* d_cells, int32_t* d_cell_index);
 
+/* Mesh decompositions of lvrhip_mesh_build (lvr2_reconstruct's --decomposition, Options.cpp:74). */
+#define LVRHIP_MESH_MC 0  /* "MC": FastBox cells */
+#define LVRHIP_MESH_PMC 1 /* "PMC", lvr2_reconstruct's default: BilinearFastBox cells */
+
+/* Marching cubes over the lattice of the last lvrhip_grid_build: FastReconstruction::getMesh
+ * (FastReconstruction.tcc:47-65, 155-165) after the erase of every cell with an invalid corner
+ * that PointsetGrid::calcDistanceValues performs (PointsetGrid.tcc:153-184); replaces the
+ * reconstruction->getMesh(mesh) call of lvr2_reconstruct (Main.cpp:424-452).  d_dist: the signed
+ * distance of each lattice vertex (nv floats, device memory: the proj of lvrhip_distances_device
+ * at the lattice's vertices); d_invalid: their invalid flags (nv bytes, device memory) or NULL
+ * (no cell erased); nv must equal the lattice's vertex count.  A cell's case is its corners with distance > 0 (FastBox.tcc:107-117), its
+ * triangles lvr2's marching-cubes table (MCTable.hpp); one mesh vertex per lattice edge with a
+ * sign change, at FastBox::calcIntersection's position as computed by the first cell (in
+ * increasing cell index order) that uses it.
+ * decomposition LVRHIP_MESH_MC: FastBox::getSurface (FastBox.tcc:206-269).
+ * decomposition LVRHIP_MESH_PMC: BilinearFastBox::getSurface (BilinearFastBox.tcc:56-137: a face
+ * with two equal corner positions is skipped), then BilinearFastBox::optimizePlanarFaces(mesh, 5)
+ * on every cell (BilinearFastBox.tcc:140-212): the end points of the border edges of a cell with
+ * one or two of them move to the centroid of their 5 nearest cloud points, once per such
+ * (cell, edge); the handle's cloud is the search tree.
+ * The reference visits its cells in the order of an unordered map, so its vertex and face
+ * numbering vary between runs; this one is fixed (vertices in order of first use, faces in cell
+ * order).  Returns the vertex and face counts; the mesh stays in the handle.  Single-scan
+ * handles only. */
+int lvrhip_mesh_build(lvrhip_t h, const float* d_dist, const uint8_t* d_invalid, size_t nv, int decomposition,
+                      size_t* nverts, size_t* nfaces);
+/* Copies of the last mesh: vertices (nverts*3 floats) and faces (nfaces*3 vertex ids, the
+ * winding of the table).  Either may be NULL.  _device: device pointers, not synchronised. */
+int lvrhip_mesh_get(lvrhip_t h, float* vertices, int32_t* faces);
+int lvrhip_mesh_get_device(lvrhip_t h, float* d_vertices, int32_t* d_faces);
+
 /* Number of points of the handle. */
 int lvrhip_size(lvrhip_t h, size_t* n);
 

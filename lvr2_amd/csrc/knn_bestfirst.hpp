@@ -706,4 +706,44 @@ struct BfMeanNormalOut {
     }
 };
 
+// The centroid of the neighbours (BilinearFastBox::optimizePlanarFaces, BilinearFastBox.tcc:172-186):
+// a zero vector plus each neighbour's position in ascending order, divided by their number.
+struct BfCentroidOut {
+    const float4* pts;  // sorted points
+    const uint32_t* qorig;
+    float* out;  // caller order, 3 floats per query
+    uint32_t* redo;
+
+    template <int K>
+    __device__ __forceinline__ void emit(int64_t i, const BfList<K>& L, int t, bool ok, float qx, float qy,
+                                         float qz) const
+    {
+        redo[i] = ok ? 0u : 1u;
+        if (!ok) return;
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        int nk = 0;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            if (j >= K - t && L.sp[j] != ~0u) {
+                const float4 p = pts[L.sp[j]];
+                cx += p.x;
+                cy += p.y;
+                cz += p.z;
+                nk++;
+            }
+        }
+        const size_t o = 3 * (size_t)qorig[i];
+        if (nk == 0) {  // nothing found: the position stays
+            out[o] = qx;
+            out[o + 1] = qy;
+            out[o + 2] = qz;
+            return;
+        }
+        const float c = (float)nk;
+        out[o] = cx / c;
+        out[o + 1] = cy / c;
+        out[o + 2] = cz / c;
+    }
+};
+
 }  // namespace lvrhip

@@ -9,6 +9,7 @@
 #include "knn_bestfirst.hpp"
 #include "adjoint.hpp"
 #include "lattice.hpp"
+#include "mesh.hpp"
 
 #include "lvrhip.h"
 
@@ -394,6 +395,37 @@ __global__ void mean_normal_rows_kernel(int64_t m, int k, int t, const int32_t* 
     out[3 * r + 2] = rz / c;
 }
 
+// the centroid of each row's neighbours, for the tile kernel's rows (the input position stays when
+// the row is empty)
+__global__ void centroid_rows_kernel(int64_t m, int k, const int32_t* __restrict__ idx,
+                                     const uint32_t* __restrict__ rank, const float4* __restrict__ pts,
+                                     const float* __restrict__ q, float* __restrict__ out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    int nk = 0;
+    for (int j = 0; j < k; j++) {
+        const int32_t id = idx[(size_t)r * k + j];
+        if (id < 0) continue;
+        const float4 p = pts[rank[id]];
+        cx += p.x;
+        cy += p.y;
+        cz += p.z;
+        nk++;
+    }
+    if (nk == 0) {
+        out[3 * r] = q[3 * r];
+        out[3 * r + 1] = q[3 * r + 1];
+        out[3 * r + 2] = q[3 * r + 2];
+        return;
+    }
+    const float c = (float)nk;
+    out[3 * r] = cx / c;
+    out[3 * r + 1] = cy / c;
+    out[3 * r + 2] = cz / c;
+}
+
 // Best-first kNN launch for t <= BF_KMAX: the list length is the smallest of 8, 10, 16, 32 that
 // holds t (10: the signed distance's usual kd).
 // Query coordinates: SoA (qs = 1) or AoS (qs = 3, qy = qx + 1, qz = qx + 2).  With a queue the
@@ -653,7 +685,8 @@ struct lvrhip_surface {
     DBuf<uint32_t> s_bsum;
     // device counters, allocated and zeroed with the handle: [r] = queries of adaptive round r
     // (r = 2..5), [6] temporary list lengths of the merged rounds and the adjoint, [7] the
-    // lattice's, [8] queries a best-first search left to the tile kernel
+    // lattice's, [8] queries a best-first search left to the tile kernel, [9] / [10] the mesh's
+    // face and vertex counts, [11] the most moves of one mesh vertex, [12] a move round's vertices
     static constexpr int CNT_SLOTS = 16;
     DBuf<uint32_t> d_cnt;
     uint32_t* counter(int i) const
@@ -1650,6 +1683,137 @@ struct lvrhip_surface {
         grid_nc = nc;
     }
 
+    // ---- marching cubes over the lattice (mesh.hpp) -------------------------------------------
+    DBuf<uint8_t> mc_case, mc_keep, mc_border;
+    DBuf<uint32_t> mc_ntri, mc_off, mc_sums, mc_fcell, mc_vals, mc_moves, mc_list;
+    DBuf<uint64_t> mc_keys;
+    DBuf<float> mesh_vtx, mc_q, mc_c;
+    DBuf<int32_t> mesh_faces;
+    int64_t mesh_nv = 0, mesh_nf = 0, mesh_moves = 0;
+
+    uint32_t read_counter(int i)
+    {
+        LVRHIP_CHECK(hipMemcpyAsync(h_pin, counter(i), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        LVRHIP_CHECK(hipStreamSynchronize(st));
+        return h_pin[0];
+    }
+
+    // decomposition: LVRHIP_MESH_MC (FastBox cells) or LVRHIP_MESH_PMC (BilinearFastBox cells and
+    // the border-edge moves with 5 neighbours, FastReconstruction.tcc:155-165)
+    void build_mesh(const float* d_dist, const uint8_t* d_inv, int decomposition)
+    {
+        require_one_scan("the mesh");
+        drop_graph();
+        mesh_nv = mesh_nf = mesh_moves = 0;
+        const int64_t nc = grid_nc;
+        if (nc == 0) return;
+        if (grid_nv * 3 >= (int64_t)0xFFFFFFFFu) throw Error(LVRHIP_ECAPACITY, "lattice too large for the mesh");
+        const int pmc = decomposition == LVRHIP_MESH_PMC ? 1 : 0;
+        mc_case.reserve(nc);
+        mc_keep.reserve(nc);
+        mc_ntri.reserve(nc);
+        mc_off.reserve(nc);
+        mc_sums.reserve(ceil_div(nc, SCAN_SEG) + 1);
+        const unsigned gc = (unsigned)ceil_div(nc, 256);
+        mc_cells_kernel<<<gc, 256, 0, st>>>(grid_cells.p, nc, grid_vtx.p, d_dist, d_inv, pmc, mc_case.p, mc_keep.p,
+                                            mc_ntri.p);
+        LVRHIP_LAUNCH_CHECK();
+        LVRHIP_CHECK(hipMemcpyAsync(mc_off.p, mc_ntri.p, (size_t)nc * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        scan_exclusive(mc_off.p, nc, mc_sums.p, st);
+        mc_total_kernel<<<1, 64, 0, st>>>(mc_off.p, mc_ntri.p, nc, counter(9));
+        LVRHIP_LAUNCH_CHECK();
+        const int64_t nf = read_counter(9);
+        if (nf == 0) return;
+        const int64_t m = 3 * nf;  // triangle corners
+        if (m >= (int64_t)0xFFFFFFFFu) throw Error(LVRHIP_ECAPACITY, "mesh too large");
+        mc_keys.reserve(m);
+        mc_vals.reserve(m);
+        mc_fcell.reserve(nf);
+        g_k1.reserve(m);
+        g_v1.reserve(m);
+        mc_emit_kernel<<<gc, 256, 0, st>>>(grid_cells.p, nc, mc_case.p, mc_keep.p, mc_off.p, mc_keys.p, mc_vals.p,
+                                           mc_fcell.p);
+        LVRHIP_LAUNCH_CHECK();
+        lat_sort(mc_keys.p, mc_vals.p, m, 32);  // by lattice edge, stable in corner order
+        // one vertex per lattice edge: the group of its first use
+        g_flags.reserve(m);
+        lat_first_flags_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(mc_keys.p, m, g_flags.p);
+        LVRHIP_LAUNCH_CHECK();
+        const int64_t nb = ceil_div(m, (int64_t)LAT_TILE);
+        s_bsum.reserve(nb + 1);
+        flag_block_sums_kernel<<<(unsigned)nb, CF_THREADS, 0, st>>>(g_flags.p, m, nullptr, s_bsum.p);
+        LVRHIP_LAUNCH_CHECK();
+        scan_exclusive_kernel<<<1, 1024, 0, st>>>(s_bsum.p, nb, counter(10));
+        LVRHIP_LAUNCH_CHECK();
+        g_gid.reserve(m);
+        lat_group_ids_kernel<<<(unsigned)nb, LAT_THREADS, 0, st>>>(g_flags.p, m, s_bsum.p, g_gid.p);
+        LVRHIP_LAUNCH_CHECK();
+        const int64_t nv = read_counter(10);
+        // vertex ids in first-use order
+        g_first.reserve(nv);
+        g_vidx.reserve(nv);
+        lat_first_order_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(g_flags.p, g_gid.p, mc_vals.p, m, g_first.p,
+                                                                          g_vidx.p);
+        LVRHIP_LAUNCH_CHECK();
+        lat_sort(g_first.p, g_vidx.p, nv, 32);
+        g_vid.reserve(nv);
+        lat_vertex_ids_kernel<<<(unsigned)ceil_div(nv, 256), 256, 0, st>>>(g_vidx.p, nv, g_vid.p);
+        LVRHIP_LAUNCH_CHECK();
+        mesh_faces.reserve(m);
+        lat_cells_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(g_gid.p, mc_vals.p, g_vid.p, m, mesh_faces.p);
+        LVRHIP_LAUNCH_CHECK();
+        mesh_vtx.reserve(3 * nv);
+        mc_vertices_kernel<<<(unsigned)ceil_div(nv, 256), 256, 0, st>>>(g_first.p, nv, mc_fcell.p, mc_off.p, mc_case.p,
+                                                                        mc_keep.p, grid_cells.p, grid_vtx.p, d_dist,
+                                                                        mesh_vtx.p);
+        LVRHIP_LAUNCH_CHECK();
+        LVRHIP_CHECK(hipStreamSynchronize(st));
+        mesh_nv = nv;
+        mesh_nf = nf;
+        if (pmc) optimize_planar_faces(nc, 5);
+    }
+
+    // BilinearFastBox::optimizePlanarFaces(mesh, kc) over every cell (BilinearFastBox.tcc:140-212):
+    // border edges from the edge multiplicities, per-vertex move counts, then one round of centroid
+    // moves per count level.
+    void optimize_planar_faces(int64_t nc, int kc)
+    {
+        const int64_t nf = mesh_nf, nv = mesh_nv, m = 3 * nf;
+        mc_keys.reserve(m);
+        mc_vals.reserve(m);
+        g_k1.reserve(m);
+        g_v1.reserve(m);
+        mc_face_edges_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(mesh_faces.p, nf, mc_keys.p, mc_vals.p);
+        LVRHIP_LAUNCH_CHECK();
+        lat_sort(mc_keys.p, mc_vals.p, m);
+        mc_border.reserve(m);
+        mc_border_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(mc_keys.p, mc_vals.p, m, mc_border.p);
+        LVRHIP_LAUNCH_CHECK();
+        mc_moves.reserve(nv);
+        LVRHIP_CHECK(hipMemsetAsync(mc_moves.p, 0, (size_t)nv * sizeof(uint32_t), st));
+        LVRHIP_CHECK(hipMemsetAsync(counter(11), 0, 2 * sizeof(uint32_t), st));
+        mc_moves_kernel<<<(unsigned)ceil_div(nc, 256), 256, 0, st>>>(mc_off.p, mc_ntri.p, nc, mc_border.p,
+                                                                    mesh_faces.p, mc_moves.p, counter(11));
+        LVRHIP_LAUNCH_CHECK();
+        const uint32_t rounds = read_counter(11);
+        for (uint32_t r = 1; r <= rounds; r++) {
+            LVRHIP_CHECK(hipMemsetAsync(counter(12), 0, sizeof(uint32_t), st));
+            mc_list.reserve(nv);
+            mc_q.reserve(3 * nv);
+            mc_round_list_kernel<<<(unsigned)ceil_div(nv, 256), 256, 0, st>>>(mc_moves.p, nv, r, mesh_vtx.p,
+                                                                             mc_list.p, mc_q.p, counter(12));
+            LVRHIP_LAUNCH_CHECK();
+            const int64_t mr = read_counter(12);
+            if (mr == 0) break;
+            mc_c.reserve(3 * mr);
+            centroids_device(mc_q.p, mr, kc, mc_c.p);
+            mc_round_scatter_kernel<<<(unsigned)ceil_div(mr, 256), 256, 0, st>>>(mc_list.p, mr, mc_c.p, mesh_vtx.p);
+            LVRHIP_LAUNCH_CHECK();
+            mesh_moves += mr;
+        }
+        LVRHIP_CHECK(hipStreamSynchronize(st));
+    }
+
     // Reverse pass of distances: d_grad (n x 3, input order) += d(sum_v w_v proj_v)/d(points)
     // with neighbour sets, adaptive k and flip signs held fixed.
     void distances_backward_device(const float* d_q, int64_t m, const float* d_w, float* d_grad)
@@ -1839,6 +2003,40 @@ struct lvrhip_surface {
             LVRHIP_LAUNCH_CHECK();
         }
         check_err();
+    }
+
+    // The centroid of the kc cloud points nearest to each of m device positions (the vertex move
+    // of BilinearFastBox::optimizePlanarFaces, BilinearFastBox.tcc:168-204); d_out in caller order.
+    void centroids_device(const float* d_q, int64_t m, int kc, float* d_out)
+    {
+        require_one_scan("the mesh");
+        if (m == 0) return;
+        if (n == 0) throw Error(LVRHIP_ESTATE, "the mesh needs a non-empty point cloud");
+        const int t0 = clamp_k(kc);
+        if (best_first(t0)) {
+            sort_queries(d_q, m);
+            s_redo.reserve(m);
+            BfCentroidOut o{pts4.p, s_v0.p, d_out, s_redo.p};
+            enqueue_bf(m, t0, BF_EMPTY, o, 7);
+            if (bf_redo_count() == 0) {
+                check_err();
+                return;
+            }
+        }
+        DBuf<int32_t> idx;
+        idx.reserve((size_t)m * t0);
+        const int saved = search_mode;
+        search_mode = LVRHIP_SEARCH_TILE;
+        try {
+            knn_device(d_q, m, t0, idx.p, nullptr);
+        } catch (...) {
+            search_mode = saved;
+            throw;
+        }
+        search_mode = saved;
+        centroid_rows_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(m, t0, idx.p, rank.p, pts4.p, d_q, d_out);
+        LVRHIP_LAUNCH_CHECK();
+        LVRHIP_CHECK(hipStreamSynchronize(st));
     }
 
     // PointsetSurface::getInterpolatedNormal (PointsetSurface.tcc:61-74) at m device positions.
@@ -2338,6 +2536,52 @@ int lvrhip_grid_get(lvrhip_t h, float* vertices, int32_t* cells, int32_t* cell_i
                                         hipMemcpyDeviceToHost, h->st));
         if (cell_index && h->grid_nc)
             LVRHIP_CHECK(hipMemcpyAsync(cell_index, h->grid_cidx.p, 3 * (size_t)h->grid_nc * sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, h->st));
+        LVRHIP_CHECK(hipStreamSynchronize(h->st));
+    });
+}
+
+int lvrhip_mesh_build(lvrhip_t h, const float* d_dist, const uint8_t* d_invalid, size_t nv, int decomposition,
+                      size_t* nverts, size_t* nfaces)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        REQUIRE(nv == (size_t)h->grid_nv, "the distances must hold one value per lattice vertex (" +
+                                              std::to_string(h->grid_nv) + "), got " + std::to_string(nv));
+        REQUIRE(h->grid_nc == 0 || d_dist, "NULL distance pointer");
+        REQUIRE(decomposition == LVRHIP_MESH_MC || decomposition == LVRHIP_MESH_PMC,
+                "decomposition must be LVRHIP_MESH_MC or LVRHIP_MESH_PMC");
+        DeviceGuard dg(h->device);
+        h->build_mesh(d_dist, d_invalid, decomposition);
+        if (nverts) *nverts = (size_t)h->mesh_nv;
+        if (nfaces) *nfaces = (size_t)h->mesh_nf;
+    });
+}
+
+int lvrhip_mesh_get_device(lvrhip_t h, float* d_vertices, int32_t* d_faces)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        DeviceGuard dg(h->device);
+        if (d_vertices && h->mesh_nv)
+            LVRHIP_CHECK(hipMemcpyAsync(d_vertices, h->mesh_vtx.p, 3 * (size_t)h->mesh_nv * sizeof(float),
+                                        hipMemcpyDeviceToDevice, h->st));
+        if (d_faces && h->mesh_nf)
+            LVRHIP_CHECK(hipMemcpyAsync(d_faces, h->mesh_faces.p, 3 * (size_t)h->mesh_nf * sizeof(int32_t),
+                                        hipMemcpyDeviceToDevice, h->st));
+    });
+}
+
+int lvrhip_mesh_get(lvrhip_t h, float* vertices, int32_t* faces)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        DeviceGuard dg(h->device);
+        if (vertices && h->mesh_nv)
+            LVRHIP_CHECK(hipMemcpyAsync(vertices, h->mesh_vtx.p, 3 * (size_t)h->mesh_nv * sizeof(float),
+                                        hipMemcpyDeviceToHost, h->st));
+        if (faces && h->mesh_nf)
+            LVRHIP_CHECK(hipMemcpyAsync(faces, h->mesh_faces.p, 3 * (size_t)h->mesh_nf * sizeof(int32_t),
                                         hipMemcpyDeviceToHost, h->st));
         LVRHIP_CHECK(hipStreamSynchronize(h->st));
     });

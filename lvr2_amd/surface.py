@@ -293,6 +293,48 @@ class HipSurface:
                                     ci.ctypes.data_as(C.c_void_p)))
         return v, c, ci
 
+    def mesh(self, dist, invalid=None, decomposition: str = "PMC", device: bool = False):
+        """FastReconstruction::getMesh over the last lattice (``grid``) given the signed distance of
+        every lattice vertex (device tensor, nv float32) and their invalid flags (device tensor, nv
+        uint8, or None).  decomposition "PMC" (lvr2_reconstruct's default: BilinearFastBox cells,
+        degenerate faces skipped, border vertices moved by optimizePlanarFaces with this surface's
+        points) or "MC" (FastBox cells).  Returns (vertices (V, 3) float32, faces (F, 3) int32),
+        torch device tensors with device=True."""
+        import torch
+
+        modes = {"MC": 0, "PMC": 1}
+        if decomposition not in modes:
+            raise ValueError(f"decomposition must be one of {sorted(modes)}")
+        if not _is_device_tensor(dist) or dist.dtype != torch.float32 or not dist.is_contiguous() or dist.ndim != 1:
+            raise ValueError("dist must be a contiguous 1-D float32 device tensor (one value per lattice vertex)")
+        if invalid is not None:
+            if (not _is_device_tensor(invalid) or invalid.dtype != torch.uint8 or not invalid.is_contiguous()
+                    or tuple(invalid.shape) != tuple(dist.shape)):
+                raise ValueError("invalid must be a contiguous uint8 device tensor shaped like dist")
+        nv, nf = C.c_size_t(), C.c_size_t()
+        check(lib().lvrhip_mesh_build(self._h, C.c_void_p(dist.data_ptr()),
+                                      C.c_void_p(invalid.data_ptr() if invalid is not None else 0),
+                                      int(dist.shape[0]), modes[decomposition], C.byref(nv), C.byref(nf)))
+        nv, nf = nv.value, nf.value
+        if device:
+            v = torch.empty((nv, 3), dtype=torch.float32, device=dist.device)
+            f = torch.empty((nf, 3), dtype=torch.int32, device=dist.device)
+            check(lib().lvrhip_mesh_get_device(self._h, C.c_void_p(v.data_ptr()), C.c_void_p(f.data_ptr())))
+            torch.cuda.synchronize(dist.device)
+            return v, f
+        v = np.empty((nv, 3), np.float32)
+        f = np.empty((nf, 3), np.int32)
+        check(lib().lvrhip_mesh_get(self._h, v.ctypes.data_as(C.c_void_p), f.ctypes.data_as(C.c_void_p)))
+        return v, f
+
+    def reconstruct(self, voxel: float, extrude: bool = True, decomposition: str = "PMC", device: bool = False):
+        """The reconstruction after the normals, on the device: the PointsetGrid lattice, the signed
+        distances at its vertices with the invalid rule, and the mesh over the cells without an
+        invalid corner (PointsetGrid.tcc:41-185, FastReconstruction.tcc:47-165)."""
+        v, _, _ = self.grid(voxel, extrude, device=True)
+        proj, _, inv = self.distances(v, voxel)
+        return self.mesh(proj, inv, decomposition, device=device)
+
     def adaptiveK(self) -> np.ndarray:
         """Per point (input order): the k its normal was fitted on (the adaptive loop's final k)."""
         out = np.empty(self._n, np.int32)

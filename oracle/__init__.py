@@ -47,6 +47,8 @@ def _l():
         l.orc_distance.argtypes = [vp, vp, i32, vp, i64, vp, vp]
         l.orc_grid.argtypes = [vp, i64, vp, f32, i32, C.POINTER(i64), C.POINTER(i64), vp, vp, vp]
         l.orc_calc_distance_values.argtypes = [vp, i64, f32, vp, i64, vp, vp]
+        l.orc_mesh.argtypes = [vp, i64, vp, i64, vp, vp, i32, i64, i64, C.POINTER(i64), C.POINTER(i64), vp, vp, vp]
+        l.orc_pmc_optimize.argtypes = [vp, i32, vp, i64, vp, vp, i64, C.POINTER(i64), C.POINTER(i64)]
         l.orc_eig3_f32.argtypes = [vp, vp, vp]
         l.orc_eig3_f64.argtypes = [vp, vp, vp]
         l.orc_num_threads.restype = i32
@@ -90,6 +92,16 @@ class Surface:
         d2 = np.empty((q.shape[0], k), np.float32)
         _l().orc_knn(self._s, _p(q), q.shape[0], k, _p(idx), _p(d2))
         return idx, d2
+
+    def pmc_optimize(self, vertices, faces, face_cell, kc: int = 5):
+        """BilinearFastBox::optimizePlanarFaces over every cell, sequentially: returns (moved
+        vertices, end-point moves, non-manifold edge count)."""
+        v = _f32(vertices).reshape(-1, 3).copy()
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        fc = np.ascontiguousarray(face_cell, dtype=np.int32)
+        mv, bad = C.c_int64(), C.c_int64()
+        _l().orc_pmc_optimize(self._s, kc, _p(v), v.shape[0], _p(f), _p(fc), f.shape[0], C.byref(mv), C.byref(bad))
+        return v, mv.value, bad.value
 
     def normals(self, kn: int, method: int = 3, flip=(0.0, 0.0, 0.0), subset=None, fp64: bool = False,
                 gap: bool = False):
@@ -189,6 +201,25 @@ def calc_distance_values(eucl, voxel: float, cells):
     keep = np.empty(c.shape[0], np.uint8)
     _l().orc_calc_distance_values(_p(e), e.shape[0], voxel, _p(c), c.shape[0], _p(inv), _p(keep))
     return inv, keep
+
+
+def mesh(vertices, cells, dist, invalid=None, pmc: bool = False, face_cells: bool = False):
+    """FastReconstruction::getMesh over a lattice with FastBox cells (pmc=False) or BilinearFastBox
+    cells (pmc=True, degenerate faces skipped): (vertices (V, 3) float32, faces (F, 3) int32), plus
+    the cell of every face when face_cells=True."""
+    v = _f32(vertices).reshape(-1, 3)
+    c = np.ascontiguousarray(cells, dtype=np.int32)
+    d = _f32(dist).ravel()
+    inv = None if invalid is None else np.ascontiguousarray(invalid, dtype=np.uint8)
+    nv_, nf_ = C.c_int64(), C.c_int64()
+    cap_v, cap_f = 3 * v.shape[0] + 1, 4 * c.shape[0] + 1
+    ov = np.empty((cap_v, 3), np.float32)
+    of = np.empty((cap_f, 3), np.int32)
+    oc = np.empty(cap_f, np.int32)
+    _l().orc_mesh(_p(v), v.shape[0], _p(c), c.shape[0], _p(d), _p(inv), int(pmc), cap_v, cap_f, C.byref(nv_),
+                  C.byref(nf_), _p(ov), _p(of), _p(oc))
+    out = (ov[: nv_.value].copy(), of[: nf_.value].copy())
+    return out + (oc[: nf_.value].copy(),) if face_cells else out
 
 
 def eig3(cov, fp64: bool = False):

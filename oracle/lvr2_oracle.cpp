@@ -11,6 +11,7 @@
  * Built with -ffp-contract=off so every FP32 expression rounds exactly as written.
  */
 #include "lvr2_oracle.h"
+#include "../lvr2_amd/csrc/mc_table.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -946,6 +947,151 @@ int orc_grid(const float* xyz, int64_t n, const float bb[6], float voxel, int ex
             out_cell_idx[3 * i + 1] = c.y;
             out_cell_idx[3 * i + 2] = c.z;
         }
+    return 0;
+}
+
+int orc_mesh(const float* vtx, int64_t nv, const int32_t* cells, int64_t nc, const float* dist,
+             const uint8_t* invalid, int pmc, int64_t cap_v, int64_t cap_f, int64_t* out_nv, int64_t* out_nf,
+             float* out_vtx, int32_t* out_faces, int32_t* out_face_cell)
+{
+    // FastReconstruction::getMesh over FastBox cells (FastReconstruction.tcc:47-65), the cells in
+    // increasing index order (the lattice's order; lvr2 iterates an unordered map), after the
+    // erase of the cells with an invalid corner (PointsetGrid.tcc:153-184).
+    static const uint64_t table[256] = {LVRHIP_MC_TABLE};
+    // lvr2's edge numbering (FastBox::getIntersections, FastBox.tcc:159-202): the two corners
+    // along the edge, low then high, and the corner the other two coordinates are read from
+    static const int edge_lo[12] = {0, 1, 3, 0, 4, 5, 7, 4, 0, 1, 3, 2};
+    static const int edge_hi[12] = {1, 2, 2, 3, 5, 6, 6, 7, 4, 5, 7, 6};
+    static const int edge_axis[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+    static const int edge_src[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 3, 2};
+    // a vertex created by one cell is handed to the cells that share its lattice edge
+    // (FastBox.tcc:236-253 through neighbor_table / neighbor_vertex_table): keyed here by the
+    // edge's low lattice vertex and axis
+    std::unordered_map<uint64_t, int32_t> shared;
+    int64_t nverts = 0, nfaces = 0;
+    for (int64_t c = 0; c < nc; c++) {
+        const int32_t* cv = cells + 8 * c;
+        bool bad = false;
+        for (int v = 0; v < 8; v++) bad = bad || (invalid && invalid[cv[v]]);
+        if (bad) continue;  // erased (PointsetGrid.tcc:153-184) / FastBox::isInvalid
+        float corner[8][3], d[8];
+        for (int v = 0; v < 8; v++) {
+            for (int a = 0; a < 3; a++) corner[v][a] = vtx[3 * (size_t)cv[v] + a];
+            d[v] = dist[cv[v]];
+        }
+        int index = 0;  // FastBox::getIndex
+        for (int v = 0; v < 8; v++)
+            if (d[v] > 0) index |= 1 << v;
+        // the cell's twelve edge positions (FastBox::getIntersections)
+        float pos[12][3];
+        for (int e = 0; e < 12; e++) {
+            const int lo = edge_lo[e], hi = edge_hi[e], ax = edge_axis[e];
+            // FastBox::calcIntersection (FastBox.tcc:140-160)
+            const float x1 = corner[lo][ax], x2 = corner[hi][ax], d1 = d[lo], d2 = d[hi];
+            float x;
+            if (d1 == 0)
+                x = x1;
+            else if (d2 == 0)
+                x = x2;
+            else if ((d1 < 0 && d2 > 0) || (d2 < 0 && d1 > 0))
+                x = x2 - d2 * (x1 - x2) / (d1 - d2);
+            else
+                x = 0;
+            for (int a = 0; a < 3; a++) pos[e][a] = corner[edge_src[e]][a];
+            pos[e][ax] = x;
+        }
+        const uint64_t w = table[index];
+        const int ntri = (int)(w >> 60);
+        for (int t = 0; t < ntri; t++) {
+            int es[3];
+            for (int b = 0; b < 3; b++) es[b] = (int)((w >> (4 * (3 * t + b))) & 0xF);
+            if (pmc) {
+                // BilinearFastBox::getSurface (BilinearFastBox.tcc:83-97): a face with two equal
+                // corner positions is skipped before any of its vertices is created
+                auto same = [&](int a, int b) {
+                    return pos[es[a]][0] == pos[es[b]][0] && pos[es[a]][1] == pos[es[b]][1] &&
+                           pos[es[a]][2] == pos[es[b]][2];
+                };
+                if (same(0, 1) || same(0, 2) || same(2, 1)) continue;
+            }
+            int32_t tri[3];
+            for (int b = 0; b < 3; b++) {
+                const int e = es[b];
+                const uint64_t key = (uint64_t)(uint32_t)cv[edge_lo[e]] * 3 + (uint64_t)edge_axis[e];
+                auto it = shared.find(key);
+                if (it == shared.end()) {
+                    if (nverts < cap_v)
+                        for (int a = 0; a < 3; a++) out_vtx[3 * nverts + a] = pos[e][a];
+                    it = shared.emplace(key, (int32_t)nverts++).first;
+                }
+                tri[b] = it->second;
+            }
+            if (nfaces < cap_f) {
+                for (int b = 0; b < 3; b++) out_faces[3 * nfaces + b] = tri[b];
+                if (out_face_cell) out_face_cell[nfaces] = (int32_t)c;
+            }
+            nfaces++;
+        }
+    }
+    *out_nv = nverts;
+    *out_nf = nfaces;
+    return (nverts <= cap_v && nfaces <= cap_f) ? 0 : 1;
+}
+
+int orc_pmc_optimize(const orc_surface* s, int kc, float* vtx, int64_t nv, const int32_t* faces,
+                     const int32_t* face_cell, int64_t nf, int64_t* out_moves, int64_t* out_nonmanifold)
+{
+    // FastReconstruction::getMesh's BilinearFastBox tail (FastReconstruction.tcc:155-165) calling
+    // BilinearFastBox::optimizePlanarFaces(mesh, 5) (BilinearFastBox.tcc:140-212) on every cell,
+    // in cell order, literally: a cell whose own faces have one or two border edges (edges with a
+    // face on one side only, HalfEdgeMesh::isBorderEdge, HalfEdgeMesh.tcc:732-746) moves both
+    // end points of each such edge, one after the other, to the centroid of the kc points nearest
+    // to the end point's current position.
+    auto ukey = [](int32_t a, int32_t b) {
+        return a < b ? ((uint64_t)(uint32_t)a << 32) | (uint32_t)b : ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
+    };
+    std::unordered_map<uint64_t, int> faces_of, half_edges;
+    for (int64_t f = 0; f < nf; f++)
+        for (int b = 0; b < 3; b++) {
+            const int32_t u = faces[3 * f + b], v = faces[3 * f + (b + 1) % 3];
+            faces_of[ukey(u, v)]++;
+            half_edges[((uint64_t)(uint32_t)u << 32) | (uint32_t)v]++;
+        }
+    // configurations on which HalfEdgeMesh::addFace panics (an edge with more than two faces, or
+    // a half-edge used twice): counted so that a test can require there are none
+    int64_t bad = 0;
+    for (auto& e : faces_of) bad += e.second > 2;
+    for (auto& e : half_edges) bad += e.second > 1;
+    if (out_nonmanifold) *out_nonmanifold = bad;
+    int64_t moves = 0;
+    std::vector<uint32_t> ids;
+    for (int64_t f0 = 0; f0 < nf;) {
+        int64_t f1 = f0;
+        while (f1 < nf && face_cell[f1] == face_cell[f0]) f1++;
+        std::vector<std::pair<int32_t, int32_t>> border;
+        for (int64_t f = f0; f < f1; f++)
+            for (int b = 0; b < 3; b++) {
+                const int32_t u = faces[3 * f + b], v = faces[3 * f + (b + 1) % 3];
+                if (faces_of[ukey(u, v)] == 1) border.emplace_back(u, v);
+            }
+        if (border.size() == 1 || border.size() == 2) {
+            for (auto& e : border)
+                for (int32_t vid : {e.first, e.second}) {
+                    float* p = vtx + 3 * (size_t)vid;
+                    s->ksearch(p, kc, ids);
+                    const size_t nk = std::min((size_t)kc, ids.size());
+                    if (nk == 0) continue;
+                    float c[3] = {0.0f, 0.0f, 0.0f};
+                    for (uint32_t j : ids)
+                        for (int a = 0; a < 3; a++) c[a] += s->pts[3 * (size_t)j + a];
+                    for (int a = 0; a < 3; a++) p[a] = c[a] / (float)nk;
+                    moves++;
+                }
+        }
+        f0 = f1;
+    }
+    if (out_moves) *out_moves = moves;
+    (void)nv;
     return 0;
 }
 

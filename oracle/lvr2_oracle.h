@@ -96,6 +96,24 @@ int orc_calc_distance_values(const float* eucl, int64_t nv, float voxel,
                              const int32_t* cells, int64_t nc,
                              uint8_t* out_invalid, uint8_t* out_cell_keep);
 
+/* FastReconstruction::getMesh (FastReconstruction.tcc:47-65) on the lattice (vertices, cells from
+ * orc_grid) and the signed distance of each lattice vertex, after the erase of the cells with an
+ * invalid corner (invalid may be NULL): FastBox cells (FastBox.tcc:206-269) when pmc is 0,
+ * BilinearFastBox cells (BilinearFastBox.tcc:56-137: faces with two equal corner positions are
+ * skipped) when pmc is 1.  Cells in the given order; vertices numbered by creation.  Outputs up to
+ * cap_v vertices / cap_f faces (out_face_cell, optional: the cell of each face); returns 1 when
+ * either did not fit (the counts are exact either way). */
+int orc_mesh(const float* vtx, int64_t nv, const int32_t* cells, int64_t nc, const float* dist,
+             const uint8_t* invalid, int pmc, int64_t cap_v, int64_t cap_f, int64_t* out_nv, int64_t* out_nf,
+             float* out_vtx, int32_t* out_faces, int32_t* out_face_cell);
+
+/* BilinearFastBox::optimizePlanarFaces(mesh, kc) over all cells (BilinearFastBox.tcc:140-212), in
+ * place on vtx, sequentially in face-cell order with kSearch on s.  out_moves: end-point moves
+ * made; out_nonmanifold: edges with more than two faces plus half-edges used twice (0 for a mesh
+ * HalfEdgeMesh accepts). */
+int orc_pmc_optimize(const orc_surface* s, int kc, float* vtx, int64_t nv, const int32_t* faces,
+                     const int32_t* face_cell, int64_t nf, int64_t* out_moves, int64_t* out_nonmanifold);
+
 /* Symmetric 3x3 eigen-decomposition used for method 3 (Jacobi; FP32 and FP64),
  * exposed so the tests can pin it against numpy.linalg.eigh. cov: row-major 3x3.
  * evals[3], evecs row-major with eigenvector j in column j. */

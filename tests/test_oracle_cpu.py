@@ -299,3 +299,192 @@ def test_fp64_interpolation_and_distance_chain(oracle_mod):
     pf, ef = sf.distance_f64(up, 6, q)
     np.testing.assert_allclose(pf, q[:, 2].astype(np.float64), rtol=0, atol=1e-12)
     assert np.all(ef >= np.abs(q[:, 2]) - 1e-12)
+
+
+def _mc_table():
+    """The packed marching-cubes table (lvr2_amd/csrc/mc_table.h) as lists of edge triples."""
+    import re
+
+    from conftest import ROOT
+
+    txt = open(os.path.join(ROOT, "lvr2_amd", "csrc", "mc_table.h")).read()
+    words = [int(x, 16) for x in re.findall(r"0x([0-9a-f]+)ull", txt)]
+    assert len(words) == 256
+    out = []
+    for w in words:
+        n = w >> 60
+        out.append([[(w >> (4 * (3 * t + b))) & 0xF for b in range(3)] for t in range(n)])
+    return out
+
+
+# lvr2's edge numbering: the two corners of each cell edge (FastBox::getIntersections)
+MC_EDGE_CORNERS = [(0, 1), (1, 2), (3, 2), (0, 3), (4, 5), (5, 6), (7, 6), (4, 7), (0, 4), (1, 5), (3, 7), (2, 6)]
+
+
+def test_mc_table_structure():
+    """Every case's triangles use exactly its sign-change edges (an edge whose two corners lie on
+    different sides), the empty and the full case have none, and a case and its complement cut
+    the same edges."""
+    tab = _mc_table()
+    assert tab[0] == [] and tab[255] == []
+    for idx, tris in enumerate(tab):
+        inside = [(idx >> v) & 1 for v in range(8)]
+        crossing = {e for e, (a, b) in enumerate(MC_EDGE_CORNERS) if inside[a] != inside[b]}
+        used = {e for t in tris for e in t}
+        assert used == crossing, idx
+        assert all(len(set(t)) == 3 for t in tris), idx
+        assert {e for t in tab[255 - idx] for e in t} == crossing
+
+
+def _lattice_and_field(oracle_mod, pts, voxel, field):
+    s = oracle_mod.Surface(pts)
+    v, cells, ci = oracle_mod.grid(pts, s.bbox(), voxel, extrude=True)
+    return v, cells, field(v.astype(np.float64)).astype(np.float32)
+
+
+def _edges(faces):
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+    return e
+
+
+def test_mesh_of_a_sphere_is_closed_and_consistently_oriented(oracle_mod):
+    rng = np.random.default_rng(2)
+    d = rng.normal(size=(40000, 3))
+    pts = (3.0 * d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    v, cells, dist = _lattice_and_field(oracle_mod, pts, 0.25, lambda x: 3.0 - np.linalg.norm(x, axis=1))
+    mv, mf = oracle_mod.mesh(v, cells, dist)
+    assert len(mf) > 1000
+    # every directed edge once and its reverse once: a closed, consistently oriented 2-manifold
+    e = _edges(mf)
+    fwd = set(map(tuple, e))
+    assert len(fwd) == len(e)
+    assert all((b, a) in fwd for a, b in fwd)
+    euler = len(mv) - len(fwd) // 2 + len(mf)
+    assert euler == 2
+    # vertices on the sphere to the linear interpolation's accuracy, all triangles facing one way
+    r = np.linalg.norm(mv.astype(np.float64), axis=1)
+    assert np.max(np.abs(r - 3.0)) < 0.25 ** 2 / 3.0
+    p = mv.astype(np.float64)[mf]
+    n = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    outward = np.sum(n * p.mean(axis=1), 1)
+    # zero-area triangles come from lattice vertices with d = 0 exactly (several edges then meet
+    # at that corner, FastBox::calcIntersection returns it for each); the others face one way
+    assert np.mean(outward != 0) > 0.95
+    assert np.all(outward <= 0) or np.all(outward >= 0)
+
+
+def test_mesh_of_a_plane_and_the_invalid_erase(oracle_mod):
+    rng = np.random.default_rng(3)
+    xy = rng.uniform(-4, 4, size=(30000, 2))
+    h = 0.3
+    # points around the plane (a box with some height: PointsetGrid keeps only the cells whose
+    # centre lies in the cloud's bounding box)
+    pts = np.column_stack([xy, h + rng.uniform(-0.8, 0.8, 30000)]).astype(np.float32)
+    v, cells, dist = _lattice_and_field(oracle_mod, pts, 0.5, lambda x: x[:, 2] - h)
+    mv, mf = oracle_mod.mesh(v, cells, dist)
+    np.testing.assert_allclose(mv[:, 2], h, rtol=0, atol=1e-6)
+    # one vertex per crossing vertical lattice edge, two triangles per crossed cell
+    crossed = np.sum(np.any(dist[cells] > 0, axis=1) & np.any(dist[cells] <= 0, axis=1))
+    assert crossed > 100 and len(mf) == 2 * crossed
+    # erasing the cells with an invalid corner removes exactly their triangles
+    inv = (v[:, 0] > 2.0).astype(np.uint8)
+    mv2, mf2 = oracle_mod.mesh(v, cells, dist, inv)
+    kept = ~np.any(inv[cells] != 0, axis=1)
+    crossed2 = np.sum(kept & np.any(dist[cells] > 0, axis=1) & np.any(dist[cells] <= 0, axis=1))
+    assert len(mf2) == 2 * crossed2 and crossed2 < crossed
+    assert np.all(mv2[:, 0] <= 2.0 + 0.5)
+
+
+def _sphere_lattice(oracle_mod):
+    rng = np.random.default_rng(2)
+    d = rng.normal(size=(40000, 3))
+    pts = (3.0 * d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    v, cells, dist = _lattice_and_field(oracle_mod, pts, 0.25, lambda x: 3.0 - np.linalg.norm(x, axis=1))
+    return pts, v, cells, dist
+
+
+def test_pmc_skips_exactly_the_degenerate_faces(oracle_mod):
+    """BilinearFastBox drops the faces with two equal corner positions and creates no vertex for
+    them; on a lattice of exact binary positions (voxel 0.25) the rest equals the FastBox mesh."""
+    _, v, cells, dist = _sphere_lattice(oracle_mod)
+    mv, mf = oracle_mod.mesh(v, cells, dist)
+    pv, pf = oracle_mod.mesh(v, cells, dist, pmc=True)
+    p = mv[mf]
+    degenerate = (np.all(p[:, 0] == p[:, 1], axis=1) | np.all(p[:, 0] == p[:, 2], axis=1)
+                  | np.all(p[:, 1] == p[:, 2], axis=1))
+    assert degenerate.sum() > 0
+    assert len(pf) == len(mf) - degenerate.sum()
+    # the same faces, as position triples in the same order
+    np.testing.assert_array_equal(pv[pf], p[~degenerate])
+    # and only the vertices those faces use
+    used = np.unique(mf[~degenerate])
+    assert len(pv) == len(used)
+    np.testing.assert_array_equal(np.sort(pv.view("V12").ravel()), np.sort(mv[used].view("V12").ravel()))
+    # no zero-area face is left
+    q = pv.astype(np.float64)[pf]
+    assert np.all(np.linalg.norm(np.cross(q[:, 1] - q[:, 0], q[:, 2] - q[:, 0]), axis=1) > 0)
+
+
+def test_pmc_optimize_on_a_closed_surface_moves_nothing(oracle_mod):
+    """A sphere of radius 2.99 meets no lattice vertex: no face is dropped, the surface is
+    closed, no edge is a border edge and optimizePlanarFaces moves nothing.  (With radius 3 the
+    dropped zero-area faces open holes, whose rims do move.)"""
+    pts, v, cells, dist3 = _sphere_lattice(oracle_mod)
+    dist = (2.99 - np.linalg.norm(v.astype(np.float64), axis=1)).astype(np.float32)
+    assert np.all(dist != 0)
+    pv, pf, fc = oracle_mod.mesh(v, cells, dist, pmc=True, face_cells=True)
+    mv, mf = oracle_mod.mesh(v, cells, dist)
+    np.testing.assert_array_equal(pf, mf)
+    s = oracle_mod.Surface(pts)
+    ov, moves, bad = s.pmc_optimize(pv, pf, fc)
+    assert bad == 0 and moves == 0
+    np.testing.assert_array_equal(ov, pv)
+    pv3, pf3, fc3 = oracle_mod.mesh(v, cells, dist3, pmc=True, face_cells=True)
+    assert s.pmc_optimize(pv3, pf3, fc3)[1] > 0
+
+
+def _brute_centroid(pts, q, k):
+    """The centroid of the k nearest points under the contract ordering, in fp32 as lvr2 sums."""
+    d = pts[None, :, :] - q[:, None, :]
+    d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    out = np.empty_like(q)
+    for i in range(len(q)):
+        order = np.lexsort((np.arange(len(pts)), d2[i]))[:k]
+        c = np.zeros(3, np.float32)
+        for j in order:
+            c = c + pts[j]
+        out[i] = c / np.float32(len(order))
+    return out
+
+
+def test_pmc_optimize_equals_the_per_vertex_move_count_form(oracle_mod):
+    """The sequential optimizePlanarFaces loop equals moving every vertex c times, c = its number
+    of (cell with one or two border edges, border edge) pairs, each move the brute-force centroid
+    of its 5 nearest points: the form the device runs."""
+    rng = np.random.default_rng(5)
+    xy = rng.uniform(-2, 2, size=(6000, 2))
+    pts = np.column_stack([xy, 0.3 + rng.uniform(-0.6, 0.6, 6000)]).astype(np.float32)
+    v, cells, dist = _lattice_and_field(oracle_mod, pts, 0.5,
+                                        lambda x: x[:, 2] - 0.3 - 0.2 * np.sin(2 * x[:, 0]))
+    inv = (np.hypot(v[:, 0] - 0.5, v[:, 1]) < 0.8).astype(np.uint8)  # a hole in the middle
+    pv, pf, fc = oracle_mod.mesh(v, cells, dist, inv, pmc=True, face_cells=True)
+    s = oracle_mod.Surface(pts)
+    ov, moves, bad = s.pmc_optimize(pv, pf, fc)
+    assert bad == 0 and moves > 50
+    # the per-vertex form
+    e = np.sort(np.stack([pf, np.roll(pf, -1, axis=1)], axis=2).reshape(-1, 2), axis=1)
+    key = e[:, 0].astype(np.int64) << 32 | e[:, 1]
+    _, inv_idx, cnt = np.unique(key, return_inverse=True, return_counts=True)
+    border = (cnt[inv_idx] == 1).reshape(-1, 3)
+    per_cell = np.bincount(fc, weights=border.sum(1), minlength=len(cells)).astype(int)
+    c = np.zeros(len(pv), np.int64)
+    for f in np.nonzero(np.isin(per_cell[fc], (1, 2)))[0]:
+        for b in np.nonzero(border[f])[0]:
+            c[pf[f, b]] += 1
+            c[pf[f, (b + 1) % 3]] += 1
+    assert c.sum() == moves
+    ref = pv.copy()
+    for r in range(1, c.max() + 1):
+        sel = np.nonzero(c >= r)[0]
+        ref[sel] = _brute_centroid(pts, ref[sel], 5)
+    np.testing.assert_array_equal(ov, ref)
