@@ -74,3 +74,73 @@ def test_out_of_memory_contract(tmp_path):
                        timeout=120, env=env)
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
     assert "out of memory" in r.stdout
+
+
+KS_EXE = os.path.join(ROOT, "tests", "cpp", "hipksearch_check")
+
+
+def test_pointset_surface_dropin(tmp_path, oracle_mod):
+    """lvr2::HipKSearchSurface (the PointsetSurface-side class) run by tests/cpp/hipksearch_check
+    the way PointsetGrid and FastReconstruction drive a surface.  Bars: the flip point is the fp32
+    box centroid; normals bit-equal to the Python binding's with that flip point (the same
+    library call) and the CudaKSearchSurface-shaped constructor bit-equal to those; distance(v),
+    calcDistanceValuesHip (m_distance, m_invalid, the cells left) and the search tree's kSearch /
+    radiusSearch bit-exact against the oracle."""
+    from lvr2_amd import HipSurface, scenes
+
+    if not os.path.exists(KS_EXE):
+        pytest.fail(f"{KS_EXE} is not built (run __graft_entry__.build())")
+    w = scenes.plain_small()
+    o = oracle_mod.Surface(w.points)
+    v, cells, ci = oracle_mod.grid(w.points, o.bbox(), w.voxel, extrude=True)
+    rng = np.random.default_rng(11)
+    near = w.points[::997]
+    q = np.concatenate([v[:: max(1, len(v) // 3000)], near + rng.normal(scale=5, size=near.shape)])
+    q = q.astype(np.float32)
+    k, r = 12, 9.0
+    w.points.tofile(tmp_path / "points.f32")
+    v.astype(np.float32).tofile(tmp_path / "lattice.f32")
+    cells.astype(np.int32).tofile(tmp_path / "cells.i32")
+    ci.astype(np.int32).tofile(tmp_path / "cell_index.i32")
+    q.tofile(tmp_path / "queries.f32")
+    args = [KS_EXE, "run", str(tmp_path / "points.f32"), str(len(w.points)), str(w.kn), str(w.ki), str(w.kd),
+            repr(float(w.voxel)), str(tmp_path / "lattice.f32"), str(len(v)), str(tmp_path / "cells.i32"),
+            str(tmp_path / "cell_index.i32"), str(len(cells)), str(tmp_path / "queries.f32"), str(len(q)), str(k),
+            repr(r), str(tmp_path)]
+    print("running hipksearch_check:", len(w.points), "points", len(v), "vertices", len(q), "queries", flush=True)
+    res = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, (res.returncode, res.stdout, res.stderr)
+    # init(): the bounding-box centroid, in fp32
+    lo, hi = w.points.min(0), w.points.max(0)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "flip.f32", np.float32), lo + np.float32(0.5) * (hi - lo))
+    flip = np.fromfile(tmp_path / "flip.f32", np.float32)
+    # calculateSurfaceNormals = estimate + interpolate, stored in the point buffer
+    nrm = np.fromfile(tmp_path / "normals.f32", np.float32).reshape(-1, 3)
+    s = HipSurface(w.points)
+    s.setKn(w.kn)
+    s.setKi(w.ki)
+    s.setFlippoint(*(float(c) for c in flip))
+    s.calculateNormals()
+    s.interpolateNormals()
+    np.testing.assert_array_equal(nrm, s.getNormals())
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "cuda_normals.f32", np.float32).reshape(-1, 3), nrm)
+    # distance(v): proj, eucl bit-exact given those normals
+    d = np.fromfile(tmp_path / "dist_single.f32", np.float32).reshape(-1, 2)
+    op, oe = o.distance(nrm, w.kd, q)
+    np.testing.assert_array_equal(d[:, 0], op)
+    np.testing.assert_array_equal(d[:, 1], oe)
+    # calcDistanceValuesHip
+    gp, ge = o.distance(nrm, w.kd, v)
+    oinv, keep = oracle_mod.calc_distance_values(ge, w.voxel, cells)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "grid_dist.f32", np.float32), gp)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "grid_inv.u8", np.uint8), oinv)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "kept.i32", np.int32), np.nonzero(keep)[0])
+    assert 0 < keep.sum() < len(keep)
+    # the search tree
+    oi, od = o.knn(q, k)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "knn.i32", np.int32).reshape(-1, k), oi)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "knn_d2.f32", np.float32).reshape(-1, k), od)
+    ri, rd, rc = o.radius(q, k, r)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "rad.i32", np.int32).reshape(-1, k), ri)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "rad_d2.f32", np.float32).reshape(-1, k), rd)
+    assert rc.max() == k and ((rc > 0) & (rc < k)).any()
