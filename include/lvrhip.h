@@ -239,6 +239,11 @@ int lvrhip_get_adaptive_k(lvrhip_t h, int32_t* out_k);
  * vertex and cell sets).  Returns the vertex and cell counts; the lattice stays in the handle
  * (single-scan handles only).  Cell indices must lie within +-2^20. */
 int lvrhip_grid_build(lvrhip_t h, float voxel, int extrude, size_t* nv, size_t* nc);
+/* The same lattice over a given bounding box (box: min x, y, z, max x, y, z) instead of the
+ * cloud's: PointsetGrid's bb argument (PointsetGrid.tcc:41-46), which LargeScaleReconstruction
+ * sets to the chunk's box plus its overlap (LargeScaleReconstruction.tcc:207, 633).  Cells still
+ * come from the cloud's points; a cell is kept when its centre lies in box. */
+int lvrhip_grid_build_box(lvrhip_t h, float voxel, int extrude, const float* box, size_t* nv, size_t* nc);
 /* Copies of the last lattice: vertices (nv*3 floats, the positions lvrhip_distances takes),
  * cells (nc*8 vertex ids per cell, corner order of box_creation_table), cell index (nc*3 int32).
  * Any output may be NULL.  _device: device pointers, on the handle's stream, not synchronised. */
@@ -271,6 +276,10 @@ int lvrhip_grid_get_device(lvrhip_t h, float* d_vertices, int32_t* d_cells, int3
  * handles only. */
 int lvrhip_mesh_build(lvrhip_t h, const float* d_dist, const uint8_t* d_invalid, size_t nv, int decomposition,
                       size_t* nverts, size_t* nfaces);
+/* lvrhip_mesh_build with the distances and invalid flags in host memory (nv floats / nv bytes,
+ * invalid may be NULL). */
+int lvrhip_mesh_build_host(lvrhip_t h, const float* dist, const uint8_t* invalid, size_t nv, int decomposition,
+                           size_t* nverts, size_t* nfaces);
 /* Copies of the last mesh: vertices (nverts*3 floats) and faces (nfaces*3 vertex ids, the
  * winding of the table).  Either may be NULL.  _device: device pointers, not synchronised. */
 int lvrhip_mesh_get(lvrhip_t h, float* vertices, int32_t* faces);

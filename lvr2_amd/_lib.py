@@ -59,8 +59,12 @@ _SIGS = {
                                        C.c_void_p, C.c_void_p]),
     "lvrhip_interpolated_normals": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "lvrhip_set_search_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "lvrhip_grid_build_box": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_void_p, C.POINTER(C.c_size_t),
+                                        C.POINTER(C.c_size_t)]),
     "lvrhip_mesh_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
                                     C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "lvrhip_mesh_build_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                         C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "lvrhip_mesh_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "lvrhip_mesh_get_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "lvrhip_distances_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_float, C.c_void_p,

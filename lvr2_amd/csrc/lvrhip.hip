@@ -29,11 +29,15 @@ thread_local std::string g_last_error;
 
 // Every device allocation of the library.  With LVRHIP_FAIL_ALLOC_AFTER=n in the environment
 // it refuses, on the host, every allocation after the first n made since the variable took its
-// current value, returning hipErrorOutOfMemory as a full device does: callers can exercise the
-// out-of-memory contract (the error text LargeScaleReconstruction.tcc:656-662 matches) without
-// filling a device.
+// current value; with LVRHIP_FAIL_ALLOC_ABOVE=b every single allocation of more than b bytes (a
+// device with less memory).  A refusal returns hipErrorOutOfMemory as a full device does: callers
+// can exercise the out-of-memory contract (the error text LargeScaleReconstruction.tcc:656-662
+// matches) without filling a device.
 hipError_t dev_malloc(void** p, size_t bytes)
 {
+    if (const char* v = std::getenv("LVRHIP_FAIL_ALLOC_ABOVE")) {
+        if ((unsigned long long)bytes > std::strtoull(v, nullptr, 10)) return hipErrorOutOfMemory;
+    }
     if (const char* v = std::getenv("LVRHIP_FAIL_ALLOC_AFTER")) {
         static std::mutex mu;
         static std::string seen;
@@ -1581,24 +1585,34 @@ struct lvrhip_surface {
         return c;
     }
 
-    void build_grid(float voxel, bool extrude)
+    // box: the grid's bounding box (min xyz, max xyz), or NULL for the cloud's own box
+    void build_grid(float voxel, bool extrude, const float* box = nullptr)
     {
         require_one_scan("the lattice");
         if (!(voxel > 0.0f) || std::isinf(voxel)) throw Error(LVRHIP_EINVAL, "voxel size must be positive");
         drop_graph();
         grid_nv = grid_nc = 0;
         if (n == 0) return;
-        // the cloud's box (the surface's bounding box, PointsetSurface.tcc:52-56)
-        uint32_t enc[6];
-        LVRHIP_CHECK(hipMemcpyAsync(enc, sbox.p, sizeof(enc), hipMemcpyDeviceToHost, st));
-        LVRHIP_CHECK(hipStreamSynchronize(st));
         LatBox b;
-        for (int j = 0; j < 6; j++) {
-            const uint32_t e = enc[j];
-            uint32_t bits = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
-            float f;
-            std::memcpy(&f, &bits, 4);
-            (j < 3 ? b.lo[j] : b.hi[j - 3]) = f;
+        if (box) {
+            for (int j = 0; j < 3; j++) {
+                if (!(box[j] <= box[3 + j]) || std::isinf(box[j]) || std::isinf(box[3 + j]))
+                    throw Error(LVRHIP_EINVAL, "the grid box must be finite with min <= max");
+                b.lo[j] = box[j];
+                b.hi[j] = box[3 + j];
+            }
+        } else {
+            // the cloud's box (the surface's bounding box, PointsetSurface.tcc:52-56)
+            uint32_t enc[6];
+            LVRHIP_CHECK(hipMemcpyAsync(enc, sbox.p, sizeof(enc), hipMemcpyDeviceToHost, st));
+            LVRHIP_CHECK(hipStreamSynchronize(st));
+            for (int j = 0; j < 6; j++) {
+                const uint32_t e = enc[j];
+                uint32_t bits = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
+                float f;
+                std::memcpy(&f, &bits, 4);
+                (j < 3 ? b.lo[j] : b.hi[j - 3]) = f;
+            }
         }
         b.voxel = voxel;
         for (int j = 0; j < 3; j++) {
@@ -2506,6 +2520,17 @@ int lvrhip_grid_build(lvrhip_t h, float voxel, int extrude, size_t* out_nv, size
     });
 }
 
+int lvrhip_grid_build_box(lvrhip_t h, float voxel, int extrude, const float* box, size_t* out_nv, size_t* out_nc)
+{
+    return guarded([&] {
+        REQUIRE(h && box, "NULL argument");
+        DeviceGuard dg(h->device);
+        h->build_grid(voxel, extrude != 0, box);
+        if (out_nv) *out_nv = (size_t)h->grid_nv;
+        if (out_nc) *out_nc = (size_t)h->grid_nc;
+    });
+}
+
 int lvrhip_grid_get_device(lvrhip_t h, float* d_vertices, int32_t* d_cells, int32_t* d_cell_index)
 {
     return guarded([&] {
@@ -2553,6 +2578,33 @@ int lvrhip_mesh_build(lvrhip_t h, const float* d_dist, const uint8_t* d_invalid,
                 "decomposition must be LVRHIP_MESH_MC or LVRHIP_MESH_PMC");
         DeviceGuard dg(h->device);
         h->build_mesh(d_dist, d_invalid, decomposition);
+        if (nverts) *nverts = (size_t)h->mesh_nv;
+        if (nfaces) *nfaces = (size_t)h->mesh_nf;
+    });
+}
+
+int lvrhip_mesh_build_host(lvrhip_t h, const float* dist, const uint8_t* invalid, size_t nv, int decomposition,
+                           size_t* nverts, size_t* nfaces)
+{
+    return guarded([&] {
+        REQUIRE(h, "NULL handle");
+        REQUIRE(nv == (size_t)h->grid_nv, "the distances must hold one value per lattice vertex (" +
+                                              std::to_string(h->grid_nv) + "), got " + std::to_string(nv));
+        REQUIRE(h->grid_nc == 0 || dist, "NULL distance pointer");
+        REQUIRE(decomposition == LVRHIP_MESH_MC || decomposition == LVRHIP_MESH_PMC,
+                "decomposition must be LVRHIP_MESH_MC or LVRHIP_MESH_PMC");
+        DeviceGuard dg(h->device);
+        DBuf<float> dd;
+        DBuf<uint8_t> di;
+        if (nv) {
+            dd.reserve(nv);
+            LVRHIP_CHECK(hipMemcpyAsync(dd.p, dist, nv * sizeof(float), hipMemcpyHostToDevice, h->st));
+            if (invalid) {
+                di.reserve(nv);
+                LVRHIP_CHECK(hipMemcpyAsync(di.p, invalid, nv, hipMemcpyHostToDevice, h->st));
+            }
+        }
+        h->build_mesh(dd.p, invalid ? di.p : nullptr, decomposition);
         if (nverts) *nverts = (size_t)h->mesh_nv;
         if (nfaces) *nfaces = (size_t)h->mesh_nf;
     });

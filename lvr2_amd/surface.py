@@ -269,11 +269,19 @@ class HipSurface:
     def synchronize(self) -> None:
         check(lib().lvrhip_synchronize(self._h))
 
-    def grid(self, voxel: float, extrude: bool = True, device: bool = False):
+    def grid(self, voxel: float, extrude: bool = True, device: bool = False, box=None):
         """The PointsetGrid lattice: (vertices (nv, 3) float32, cells (nc, 8) int32 vertex ids,
-        cell index (nc, 3) int32); torch device tensors with device=True."""
+        cell index (nc, 3) int32); torch device tensors with device=True.  box: the grid's bounding
+        box (min xyz, max xyz), default the cloud's."""
         nv, nc = C.c_size_t(), C.c_size_t()
-        check(lib().lvrhip_grid_build(self._h, float(voxel), int(bool(extrude)), C.byref(nv), C.byref(nc)))
+        if box is None:
+            check(lib().lvrhip_grid_build(self._h, float(voxel), int(bool(extrude)), C.byref(nv), C.byref(nc)))
+        else:
+            b = np.ascontiguousarray(box, dtype=np.float32).ravel()
+            if b.shape != (6,):
+                raise ValueError("box must hold 6 values: min x, y, z, max x, y, z")
+            check(lib().lvrhip_grid_build_box(self._h, float(voxel), int(bool(extrude)), b.ctypes.data_as(C.c_void_p),
+                                              C.byref(nv), C.byref(nc)))
         nv, nc = nv.value, nc.value
         if device:
             import torch
