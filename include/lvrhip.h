@@ -95,7 +95,7 @@ int lvrhip_rebuild(lvrhip_t h);
  * graph keeps the parameters (k, method, flip points, tuning) of its capture.
  * lvrhip_graph_launch replays it without synchronising; lvrhip_synchronize waits for it
  * and reports kNN capacity errors.  Any other computing call on the handle releases the
- * graph.  Returns LVRHIP_ESTATE while profiling is on. */
+ * graph.  Returns LVRHIP_ESTATE while profiling is on, LVRHIP_EINVAL for ki above 64. */
 int lvrhip_graph_capture(lvrhip_t h, float* d_out_normals, int rebuild);
 int lvrhip_graph_launch(lvrhip_t h);
 int lvrhip_synchronize(lvrhip_t h);
@@ -130,8 +130,9 @@ int lvrhip_calculate_normals(lvrhip_t h);
 /* AdaptiveKSearchSurface::interpolateSurfaceNormals (tcc:293-335): each normal averaged with
  * its ki nearest points' normals.  No-op when ki == 0.  The C++ drop-in runs it inside
  * calculateNormals, where CudaSurface interpolates (CudaSurface.cu:1143-1148; its
- * interpolateNormals, CudaSurface.cu:1255-1257, is empty).  ki up to 64 (clamped to the cloud's
- * or the smallest scan's size); larger ki returns LVRHIP_EINVAL. */
+ * interpolateNormals, CudaSurface.cu:1255-1257, is empty).  ki is clamped to the cloud's (or the
+ * smallest scan's) size.  Up to 64 the kNN pass's own ki list is ranked; a larger ki searches
+ * sorted kSearch rows (single-scan handles; LVRHIP_EINVAL on a multi-scan handle). */
 int lvrhip_interpolate_normals(lvrhip_t h);
 
 /* Use the given normals (n*3 floats, AoS, input order) instead of estimating them: the

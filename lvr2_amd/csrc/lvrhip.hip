@@ -276,6 +276,44 @@ __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
     }
 }
 
+// interpolateSurfaceNormals for ki above 64: row i (input order) holds the ki nearest of point i
+// in ascending key order (self included), so the sum runs in that order, after the point's own
+// normal, exactly as interpolate_kernel's ranked sum does.  Output at the point's sorted slot.
+__global__ void interpolate_rows_kernel(int64_t n, int k, const int32_t* __restrict__ idx,
+                                        const uint32_t* __restrict__ rank, const float* __restrict__ nx,
+                                        const float* __restrict__ ny, const float* __restrict__ nz,
+                                        float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t si = rank[i];
+    float x = nx[si], y = ny[si], z = nz[si];
+    for (int j = 0; j < k; j++) {
+        const int32_t id = idx[(size_t)i * k + j];
+        if (id < 0) continue;
+        const uint32_t s = rank[id];
+        x += nx[s];
+        y += ny[s];
+        z += nz[s];
+    }
+    normalize3(x, y, z);
+    ox[si] = x;
+    oy[si] = y;
+    oz[si] = z;
+}
+
+// the points in input order (xyz), from the sorted SoA
+__global__ void input_order_points_kernel(const float4* __restrict__ pts4, const uint32_t* __restrict__ rank,
+                                          int64_t n, float* __restrict__ q)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pts4[rank[i]];
+    q[3 * i] = p.x;
+    q[3 * i + 1] = p.y;
+    q[3 * i + 2] = p.z;
+}
+
 __global__ void gather_normals_kernel(const float* __restrict__ in, const uint32_t* __restrict__ pidx, int64_t n,
                                       float* __restrict__ nx, float* __restrict__ ny, float* __restrict__ nz)
 {
@@ -1197,10 +1235,38 @@ struct lvrhip_surface {
                          mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
     }
 
-    void check_interpolate() const
+    // ki above 64 runs through sorted kSearch rows (interpolate_rows): single-scan handles, and
+    // not inside a captured graph (the rows are allocated per call)
+    bool rows_interpolation() const { return std::min<int64_t>(ki, nscans > 1 ? min_scan : n) > 64; }
+    void check_interpolate_graph() const
     {
-        if (std::min<int64_t>(ki, nscans > 1 ? min_scan : n) > 64)
-            throw Error(LVRHIP_EINVAL, "ki above 64 is not supported");
+        if (rows_interpolation()) throw Error(LVRHIP_EINVAL, "a captured graph takes ki <= 64");
+    }
+
+    void interpolate_rows()
+    {
+        require_one_scan("interpolation with ki above 64");
+        const int k = clamp_k(ki);
+        DBuf<float> q;
+        q.reserve(3 * (size_t)n);
+        input_order_points_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(pts4.p, rank.p, n, q.p);
+        LVRHIP_LAUNCH_CHECK();
+        DBuf<int32_t> idx;
+        idx.reserve((size_t)n * k);
+        knn_device(q.p, n, k, idx.p, nullptr);
+        tx.reserve(npad);
+        ty.reserve(npad);
+        tz.reserve(npad);
+        // padding slots keep what they hold
+        LVRHIP_CHECK(hipMemcpyAsync(tx.p, nx.p, (size_t)npad * sizeof(float), hipMemcpyDeviceToDevice, st));
+        LVRHIP_CHECK(hipMemcpyAsync(ty.p, ny.p, (size_t)npad * sizeof(float), hipMemcpyDeviceToDevice, st));
+        LVRHIP_CHECK(hipMemcpyAsync(tz.p, nz.p, (size_t)npad * sizeof(float), hipMemcpyDeviceToDevice, st));
+        const int tok = prof.start(LVRHIP_T_INTERPOLATE, st);
+        interpolate_rows_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, k, idx.p, rank.p, nx.p, ny.p, nz.p,
+                                                                           tx.p, ty.p, tz.p);
+        LVRHIP_LAUNCH_CHECK();
+        prof.stop(tok, st);
+        LVRHIP_CHECK(hipStreamSynchronize(st));
     }
 
     // interpolateSurfaceNormals (tcc:293-335) from nx/ny/nz into ox/oy/oz
@@ -1227,16 +1293,19 @@ struct lvrhip_surface {
         if (!has_normals) throw Error(LVRHIP_ESTATE, "interpolate_normals called before calculate_normals");
         drop_graph();
         if (ki <= 0 || n == 0) return;
-        check_interpolate();
-        if (!(ki_valid && kiw == clamp_k(ki))) {
-            enqueue_ki_list();
-            check_err();
-            ki_valid = true;
+        if (rows_interpolation()) {
+            interpolate_rows();
+        } else {
+            if (!(ki_valid && kiw == clamp_k(ki))) {
+                enqueue_ki_list();
+                check_err();
+                ki_valid = true;
+            }
+            tx.reserve(npad);
+            ty.reserve(npad);
+            tz.reserve(npad);
+            enqueue_interpolate(tx.p, ty.p, tz.p);
         }
-        tx.reserve(npad);
-        ty.reserve(npad);
-        tz.reserve(npad);
-        enqueue_interpolate(tx.p, ty.p, tz.p);
         if (prof.on) prof.collect();
         interpolated = true;
         std::swap(nx.p, tx.p);
@@ -1294,7 +1363,7 @@ struct lvrhip_surface {
     {
         if (prof.on) throw Error(LVRHIP_ESTATE, "switch profiling off before capturing a graph");
         if (n == 0) throw Error(LVRHIP_EINVAL, "empty point cloud");
-        if (ki > 0) check_interpolate();
+        if (ki > 0) check_interpolate_graph();
         prepare_normals();
         drop_graph();
         hipStream_t saved = st;
