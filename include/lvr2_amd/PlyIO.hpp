@@ -5,7 +5,8 @@
 //     element with x, y, z, PLYIO.cpp:680-715): x, y, z as the points; nx, ny, nz as the normals
 //     when all three are present; "face" vertex_indices as triangles.  Other properties are
 //     skipped.  Formats: ascii, binary_little_endian, binary_big_endian; every PLY scalar type
-//     (char/int8 ... double/float64) for any property, list properties of any count and item type.
+//     (char/int8 ... double/float64) for any property, list properties of any item type (at most
+//     10^6 items a row).  Counts the file is too short for are refused before any allocation.
 //     Values are converted to float; indices to int32.
 //   - writePly: binary_little_endian, "vertex" with float x, y, z (and nx, ny, nz when given),
 //     "face" with a uchar-counted int list vertex_indices, as PLYIO::save writes a mesh
@@ -184,7 +185,6 @@ inline Cloud readPly(const std::string& path)
     }
     if (!have_format) throw std::runtime_error("readPly: " + path + ": no format line");
     const bool swap = fmt != Fmt::Ascii && ((fmt == Fmt::Be) != host_big_endian());
-
     // which element holds the points: "vertex" with x, y, z, else "point"
     auto has_xyz = [](const Element& e) {
         int c = 0;
@@ -198,6 +198,23 @@ inline Cloud readPly(const std::string& path)
         for (size_t i = 0; i < els.size(); i++)
             if (els[i].name == "point" && has_xyz(els[i])) src = (int)i;
     if (src < 0) throw std::runtime_error("readPly: " + path + ": no vertex or point element with x, y, z");
+
+    // counts the file cannot hold are refused before anything is allocated: a row needs at
+    // least its scalar bytes (binary) or two characters per value (ascii)
+    {
+        const std::streampos here = in.tellg();
+        in.seekg(0, std::ios::end);
+        const double left = (double)(in.tellg() - here);
+        in.seekg(here);
+        double need = 0.0;
+        for (const auto& e : els) {
+            double row = 0.0;
+            for (const auto& p : e.props) row += fmt == Fmt::Ascii ? 2.0 : (double)(p.list ? p.ctype : p.type);
+            need += row * (double)e.count;
+        }
+        if (fmt == Fmt::Ascii && need > 0.0) need -= 1.0;  // the last value needs no separator
+        if (need > left) throw std::runtime_error("readPly: " + path + ": truncated data (the header promises more)");
+    }
 
     Cloud out;
     std::vector<unsigned char> buf;
@@ -227,7 +244,8 @@ inline Cloud readPly(const std::string& path)
                 if (fmt == Fmt::Ascii) {
                     if (p.list) {
                         double cnt;
-                        if (!(in >> cnt) || cnt < 0) throw std::runtime_error("readPly: " + path + ": truncated list");
+                        if (!(in >> cnt) || cnt < 0 || cnt > 1e6)
+                            throw std::runtime_error("readPly: " + path + ": truncated list");
                         list.resize((size_t)cnt);
                         for (auto& v : list)
                             if (!(in >> v)) throw std::runtime_error("readPly: " + path + ": truncated list");
@@ -239,7 +257,9 @@ inline Cloud readPly(const std::string& path)
                         buf.resize(p.ctype);
                         if (!in.read((char*)buf.data(), p.ctype))
                             throw std::runtime_error("readPly: " + path + ": truncated list");
-                        const size_t cnt = (size_t)decode(buf.data(), p.ctype, p.ckind, swap);
+                        const double dc = decode(buf.data(), p.ctype, p.ckind, swap);
+                        if (dc < 0 || dc > 1e6) throw std::runtime_error("readPly: " + path + ": bad list count");
+                        const size_t cnt = (size_t)dc;
                         buf.resize(cnt * p.type);
                         if (cnt && !in.read((char*)buf.data(), (std::streamsize)buf.size()))
                             throw std::runtime_error("readPly: " + path + ": truncated list");
