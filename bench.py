@@ -33,9 +33,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, gpu_normals, verts, sample_pts: int, sample_q: int):
+def cpu_baseline(w, gpu_normals, verts, sample_pts: int, sample_q: int, reps: int = 3):
     """The oracle (CPU restatement of the reference path, OpenMP over all host cores) on a
-    bounded sample of the same workload."""
+    bounded sample of the same workload; every phase is the median of `reps` runs."""
     import oracle
 
     if not os.path.exists(oracle.LIB_PATH):
@@ -45,16 +45,19 @@ def cpu_baseline(w, gpu_normals, verts, sample_pts: int, sample_q: int):
     o = oracle.Surface(w.points)
     t_build = time.perf_counter() - t
     sub = np.sort(rng.choice(len(w.points), sample_pts, replace=False))
-    t = time.perf_counter()
-    o.normals(w.kn, 3, w.flip, subset=sub)
-    t_norm = time.perf_counter() - t
-    t = time.perf_counter()
-    o.interpolate(gpu_normals, w.ki, subset=sub)
-    t_int = time.perf_counter() - t
     qs = verts[np.sort(rng.choice(len(verts), min(sample_q, len(verts)), replace=False))]
-    t = time.perf_counter()
-    o.distance(gpu_normals, w.kd, qs)
-    t_sdf = time.perf_counter() - t
+
+    def med(fn):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
+    t_norm = med(lambda: o.normals(w.kn, 3, w.flip, subset=sub))
+    t_int = med(lambda: o.interpolate(gpu_normals, w.ki, subset=sub))
+    t_sdf = med(lambda: o.distance(gpu_normals, w.kd, qs))
     return {
         "mpts": sample_pts / (t_norm + t_int) / 1e6,
         "sdf_mqps": len(qs) / t_sdf / 1e6,
@@ -65,8 +68,53 @@ def cpu_baseline(w, gpu_normals, verts, sample_pts: int, sample_q: int):
         "t_sdf_s": t_sdf,
         "sample": f"{sample_pts} random points of the {len(w.points)}-point room (normals kn={w.kn} + "
                   f"interpolation ki={w.ki}, kd-tree over all points) and {len(qs)} random lattice vertices "
-                  f"(SDF kd={w.kd})",
+                  f"(SDF kd={w.kd}); median of {reps} runs per phase",
     }
+
+
+def plain_small_leg(with_cpu: bool, reps: int = 5):
+    """The reference's README workload shape: 300,000 points, kn 10, ki 10, least-squares plane
+    fit (nem 0), flip point at the origin; normals + interpolation on the GPU (median of reps)
+    and, beside it, the oracle on all 300,000 points (median of 3)."""
+    import torch
+
+    from lvr2_amd import HipSurface, scenes
+
+    w = scenes.plain_small()
+    s = HipSurface(torch.from_numpy(w.points).to("cuda:0"))
+    s.setKn(w.kn)
+    s.setKi(w.ki)
+    s.setMethod("LS")
+    s.setFlippoint(*w.flip)
+    for _ in range(2):
+        s.calculateNormals()
+        s.interpolateNormals()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        s.calculateNormals()
+        s.interpolateNormals()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    n = len(w.points)
+    out = {"workload": f"plain scene {n} pts kn{w.kn} ki{w.ki} least-squares plane (nem 0)",
+           "gpu_mpts": round(n / float(np.median(ts)) / 1e6, 2),
+           "reference_readme_mpts": {"estimate": 2.47, "estimate_and_interpolate": 1.80,
+                                     "note": "README.md:95-124, 289,252-point scan.pts on an unstated 16-thread CPU"}}
+    if with_cpu:
+        import oracle
+
+        o = oracle.Surface(w.points)
+        tc = []
+        for _ in range(3):
+            t = time.perf_counter()
+            n32, _, _ = o.normals(w.kn, 0, w.flip)
+            o.interpolate(n32, w.ki)
+            tc.append(time.perf_counter() - t)
+        out["cpu_mpts"] = round(n / float(np.median(tc)) / 1e6, 3)
+        out["cpu_cores"] = oracle.num_threads()
+    return out
 
 
 def bench_batched(args):
@@ -263,8 +311,8 @@ def main():
     ap.add_argument("--n-az", type=int, default=3200)
     ap.add_argument("--n-el", type=int, default=1250)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
-    ap.add_argument("--cpu-sample-q", type=int, default=300_000)
+    ap.add_argument("--cpu-sample", type=int, default=200_000)
+    ap.add_argument("--cpu-sample-q", type=int, default=1_000_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel (y = x * 1) over MIB MiB three times: the known-size "
                          "reference for HBM counter scaling")
@@ -371,6 +419,20 @@ def main():
     mpts = n / ((t_norm + t_int) / 1e3) / 1e6
     sdf_mqps = m / (t_sdf / 1e3) / 1e6
 
+    # the mesh after the path (not in the step): marching cubes on the step's distances
+    mesh = {}
+    for dec in ("PMC", "MC"):
+        s.mesh(out[0], out[2], dec, device=True)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            mv, mf = s.mesh(out[0], out[2], dec, device=True)
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t) * 1e3)
+        mesh[dec] = {"ms": round(float(np.median(ts)), 3), "vertices": int(mv.shape[0]), "faces": int(mf.shape[0])}
+    log(f"[bench] mesh: PMC {mesh['PMC']['ms']:.2f} ms, MC {mesh['MC']['ms']:.2f} ms, {mesh['MC']['faces']} faces")
+
     # roofline of the dominant kernel: the kNN tile kernel of the normals (fused bounding-box
     # test + moments), over its five adaptive-round launches.  Algorithmic bytes of a launch at
     # k: per query point 12 (point) + 12 k (neighbours) + 12 (normal) + 4 ki (ki list, first
@@ -389,6 +451,8 @@ def main():
     achieved = bytes_step / launches_per_step / (avg_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
+        "limiter": "instruction issue and LDS / L2 latency: VALU busy about half the SIMD cycles at two waves "
+                   "per SIMD; measured HBM traffic is a small fraction of the algorithmic bytes",
         "kernel": f"knn_tile_kernel<{lanes}, SinkNormals<{lanes}> | SinkMoments<{lanes}>> (the five adaptive-round launches)",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
@@ -404,6 +468,8 @@ def main():
     if os.path.exists(traffic_file):
         try:
             roofline["traffic"] = json.load(open(traffic_file)).get("hbm_bytes_per_launch")
+            roofline["traffic_source"] = ("profiles/traffic_knn_normals_room_4m.md: rocprofv3 --pmc FETCH_SIZE and "
+                                          "WRITE_SIZE of this kernel on this workload, scaled by a copy kernel")
         except (OSError, ValueError):
             pass
 
@@ -413,6 +479,10 @@ def main():
         gn = s.getNormals()
         cpu = cpu_baseline(w, gn, verts_t.cpu().numpy(), args.cpu_sample, args.cpu_sample_q)
         log(f"[bench] CPU baseline done: {cpu['mpts']:.3f} Mpts/s, {cpu['sdf_mqps']:.3f} M q/s on {cpu['cores']} threads")
+
+    c1 = plain_small_leg(not args.no_cpu)
+    log(f"[bench] plain scene: GPU {c1['gpu_mpts']} Mpts/s" + (f", CPU {c1['cpu_mpts']} Mpts/s" if "cpu_mpts" in c1
+                                                                 else ""))
 
     line = {
         "metric": "Mpts/s kNN+PCA normals and M SDF queries/s at kn=50, vs host CPU; % HBM peak",
@@ -434,6 +504,8 @@ def main():
                      "build": round(build_ms, 2), "lattice": round(lattice_ms, 2)},
         "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in tm.items() if v[1]},
         "adaptive_rounds": [int(x) for x in rounds],
+        "mesh": mesh,
+        "plain_small": c1,
         "roofline": roofline,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["mpts"], 4), "unit": "Mpts/s", "cores": cpu["cores"], "kind": "port",

@@ -62,6 +62,31 @@ def test_lvr2_call_sites_through_cpp_dropin(tmp_path, oracle_mod):
     np.testing.assert_array_equal(inv, oinv)
 
 
+def test_dropin_host_code_under_sanitizers(tmp_path, oracle_mod):
+    """hipsurface_check built with AddressSanitizer and UBSan on its host code (the C++ drop-in
+    header, the QueryPoint and floatArr handling), run on the GPU like the plain build."""
+    from lvr2_amd import scenes
+
+    exe = os.path.join(ROOT, "tests", "cpp", "hipsurface_check_asan")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (run __graft_entry__.build())")
+    w = scenes.plain_small(60_000)
+    o = oracle_mod.Surface(w.points)
+    v, _, _ = oracle_mod.grid(w.points, o.bbox(), w.voxel, extrude=True)
+    w.points.tofile(tmp_path / "points.f32")
+    v.astype(np.float32).tofile(tmp_path / "queries.f32")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", UBSAN_OPTIONS="print_stacktrace=1")
+    args = [exe, "run", str(tmp_path / "points.f32"), str(len(w.points)), str(w.kn), str(w.ki), str(w.kd),
+            *(repr(float(c)) for c in w.flip), repr(float(w.voxel)), str(tmp_path / "queries.f32"), str(len(v)),
+            str(tmp_path)]
+    print("running hipsurface_check_asan", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300, env=env)
+    if "Shadow memory range interleaves" in r.stderr:
+        pytest.skip("AddressSanitizer's shadow memory collides with the GPU runtime's mappings on this box")
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-4000:])
+    assert "runtime error" not in r.stderr
+
+
 def test_out_of_memory_contract(tmp_path):
     """A refused device allocation (LVRHIP_FAIL_ALLOC_AFTER, refused on the host) surfaces as
     LVRHIP_ENOMEM with "out of memory" in the C ABI and as the std::runtime_error that
