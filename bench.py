@@ -388,6 +388,16 @@ def main():
     for i in range(args.warmup):
         step()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    # the LBVH build alone (device-resident points; the rebuild keeps the handle's buffers)
+    rb = []
+    for _ in range(4):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        s.rebuild()
+        torch.cuda.synchronize()
+        rb.append((time.perf_counter() - t) * 1e3)
+    rebuild_ms = float(np.median(rb[1:]))
+    log(f"[bench] LBVH rebuild {rebuild_ms:.2f} ms")
     torch.cuda.synchronize()
     s.setProfiling(True)
     s.timings(reset=True)
@@ -501,7 +511,8 @@ def main():
                    "kn": w.kn, "ki": w.ki, "kd": w.kd, "voxel": w.voxel},
         "sdf_mqps": round(sdf_mqps, 2),
         "phase_ms": {"normals": round(t_norm, 3), "interpolate": round(t_int, 3), "sdf": round(t_sdf, 3),
-                     "build": round(build_ms, 2), "lattice": round(lattice_ms, 2)},
+                     "build": round(build_ms, 2), "lbvh_rebuild": round(rebuild_ms, 3),
+                     "lattice": round(lattice_ms, 2)},
         "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in tm.items() if v[1]},
         "adaptive_rounds": [int(x) for x in rounds],
         "mesh": mesh,

@@ -5,9 +5,9 @@
 // Here: points are sorted by 63-bit Morton code (21 bits/axis, cubic normalisation to the
 // cloud's bounding box), cut into fixed leaves of LEAF consecutive points, and a Karras
 // (2012) radix tree is built top-down over the leaves' codes, with leaf indices
-// breaking code ties; child boxes are then refitted bottom-up with one atomic flag per
-// internal node.  Each internal node stores both children's boxes, so a traversal step
-// reads one 64-byte node.
+// breaking code ties; node boxes are then range unions of leaf boxes (each node covers a
+// contiguous leaf range), level by level with no atomics.  Each internal node stores both
+// children's boxes, so a traversal step reads one 64-byte node.
 #pragma once
 
 #include "common.hpp"
@@ -342,56 +342,69 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
     if (i == 0) nodes[0].r.z = -1;
 }
 
-// Bottom-up refit: a thread per leaf climbs; the second arrival at a node writes both
-// child boxes into it (acquire/release at agent scope order the two climbers).
-__global__ __launch_bounds__(256) void refit_kernel(int nl, const int* __restrict__ leaf_parent,
-                                                    const float4* __restrict__ leaf_lo,
-                                                    const float4* __restrict__ leaf_hi, BvhNode* nodes,
-                                                    float4* node_lo, float4* node_hi, uint32_t* flags)
+// Refit without atomics.  Every node of the Karras tree covers a contiguous leaf range
+// [min(i, r.w), max(i, r.w)], so its box is the union of the leaf boxes over that range: a range
+// union, answered from the doubling table T_j[i] = union of leaves [i, i + 2^j) as
+// T_j[first] U T_j[last - 2^j + 1] with j = floor(log2(length)).  Level j is built from level j - 1
+// (two buffers in turn), then the nodes whose length has that j take their box and write it into
+// their parent's child slot; leaves write theirs first.  Every slot has one writer, and min / max
+// give the same box in any order, so the result is the bottom-up refit's.
+
+// one child's box in its parent's a / b / c fields (left: a.xyzw b.xy, right: b.zw c.xyzw)
+__device__ __forceinline__ void put_child_box(BvhNode* nd, bool left, const float4& lo, const float4& hi)
 {
-    int leaf = blockIdx.x * blockDim.x + threadIdx.x;
-    if (leaf >= nl || nl < 2) return;
-    int p = leaf_parent[leaf];
-    bool stored = false;  // this thread has stored a node's boxes (the leaf level stores none)
-    while (p >= 0) {
-        // The boxes this thread stored at the previous level must have reached memory before
-        // the counter lets the other climber (possibly on another XCD, behind another L2) read
-        // them: wait for every outstanding store, write this XCD's L2 back (the device-scope
-        // fence), wait for that too, then the acquire/release increment.  The compiler's
-        // release sequence for the atomic alone issued the increment with the stores of the
-        // loop's previous iteration and the L2 write-back still in flight.
-        if (stored) {
-            __builtin_amdgcn_s_waitcnt(0);
-            __threadfence();
-            __builtin_amdgcn_s_waitcnt(0);
-        }
-        uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        if (old == 0) return;
-        int4 r = nodes[p].r;
-        float4 l0, h0, l1, h1;
-        if (r.x < 0) {
-            l0 = leaf_lo[-1 - r.x];
-            h0 = leaf_hi[-1 - r.x];
-        } else {
-            l0 = node_lo[r.x];
-            h0 = node_hi[r.x];
-        }
-        if (r.y < 0) {
-            l1 = leaf_lo[-1 - r.y];
-            h1 = leaf_hi[-1 - r.y];
-        } else {
-            l1 = node_lo[r.y];
-            h1 = node_hi[r.y];
-        }
-        nodes[p].a = make_float4(l0.x, l0.y, l0.z, h0.x);
-        nodes[p].b = make_float4(h0.y, h0.z, l1.x, l1.y);
-        nodes[p].c = make_float4(l1.z, h1.x, h1.y, h1.z);
-        node_lo[p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.0f);
-        node_hi[p] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.0f);
-        stored = true;
-        p = r.z;
+    if (left) {
+        nd->a = make_float4(lo.x, lo.y, lo.z, hi.x);
+        nd->b.x = hi.y;
+        nd->b.y = hi.z;
+    } else {
+        nd->b.z = lo.x;
+        nd->b.w = lo.y;
+        nd->c = make_float4(lo.z, hi.x, hi.y, hi.z);
     }
+}
+
+__global__ __launch_bounds__(256) void refit_leaf_slots_kernel(int nl, const int* __restrict__ leaf_parent,
+                                                               const float4* __restrict__ leaf_lo,
+                                                               const float4* __restrict__ leaf_hi, BvhNode* nodes)
+{
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= nl || nl < 2) return;
+    const int p = leaf_parent[leaf];
+    if (p < 0) return;
+    put_child_box(&nodes[p], nodes[p].r.x == leaf_ref(leaf), leaf_lo[leaf], leaf_hi[leaf]);
+}
+
+// level j of the doubling table from level j - 1 (half = 2^(j-1)); entries i <= nl - 2 * half
+__global__ __launch_bounds__(256) void range_union_level_kernel(int nl, int half, const float4* __restrict__ lo_in,
+                                                                const float4* __restrict__ hi_in,
+                                                                float4* __restrict__ lo_out, float4* __restrict__ hi_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nl - 2 * half) return;
+    const float4 a = lo_in[i], b = lo_in[i + half], c = hi_in[i], d = hi_in[i + half];
+    lo_out[i] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+    hi_out[i] = make_float4(fmaxf(c.x, d.x), fmaxf(c.y, d.y), fmaxf(c.z, d.z), 0.0f);
+}
+
+// the internal nodes whose leaf count has floor(log2) == j: own box, and the parent's child slot
+__global__ __launch_bounds__(256) void refit_level_kernel(int nint, int j, const float4* __restrict__ tlo,
+                                                          const float4* __restrict__ thi, BvhNode* nodes,
+                                                          float4* __restrict__ node_lo, float4* __restrict__ node_hi)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nint) return;
+    const int4 r = nodes[i].r;
+    const int first = min(i, r.w), last = max(i, r.w);
+    const int len = last - first + 1;
+    if (31 - __clz(len) != j) return;
+    const int second = last - (1 << j) + 1;
+    const float4 a = tlo[first], b = tlo[second], c = thi[first], d = thi[second];
+    const float4 lo = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+    const float4 hi = make_float4(fmaxf(c.x, d.x), fmaxf(c.y, d.y), fmaxf(c.z, d.z), 0.0f);
+    node_lo[i] = lo;
+    node_hi[i] = hi;
+    if (r.z >= 0) put_child_box(&nodes[r.z], nodes[r.z].r.x == i, lo, hi);
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v)

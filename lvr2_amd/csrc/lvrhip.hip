@@ -711,6 +711,7 @@ struct lvrhip_surface {
     DBuf<BvhNode> nodes;
     DBuf<BvhNode4> nodes4;
     DBuf<int> leaf_parent;
+    DBuf<float4> ru_lo[2], ru_hi[2];  // the refit's range-union levels, in turn
     int nleaf = 0;
 
     DBuf<float> nx, ny, nz, tx, ty, tz;
@@ -930,6 +931,10 @@ struct lvrhip_surface {
         nhi.reserve(nint);
         s_flags.reserve(std::max<int64_t>(nint, n));
         s_f.reserve(MEDIAN_MAX);
+        for (int b = 0; b < 2; b++) {
+            ru_lo[b].reserve(nleaf);
+            ru_hi[b].reserve(nleaf);
+        }
     }
 
     // LBVH build from `src`, entirely on the device (no host synchronisation).
@@ -965,14 +970,26 @@ struct lvrhip_surface {
         const int nint = std::max(nleaf - 1, 1);
         fill_u32_kernel<<<(unsigned)ceil_div(nleaf, 256), 256, 0, st>>>((uint32_t*)leaf_parent.p, nleaf, ~0u);
         LVRHIP_LAUNCH_CHECK();
-        fill_u32_kernel<<<(unsigned)ceil_div(nint, 256), 256, 0, st>>>(s_flags.p, nint, 0u);
-        LVRHIP_LAUNCH_CHECK();
         if (nleaf > 1) {
             karras_kernel<<<(unsigned)ceil_div(nleaf - 1, 256), 256, 0, st>>>(s_k0.p, nleaf, nodes.p, leaf_parent.p);
             LVRHIP_LAUNCH_CHECK();
-            refit_kernel<<<(unsigned)ceil_div(nleaf, 256), 256, 0, st>>>(nleaf, leaf_parent.p, leaf_lo.p, leaf_hi.p,
-                                                                        nodes.p, nlo.p, nhi.p, s_flags.p);
+            // the refit as range unions, level by level (lbvh.hpp)
+            refit_leaf_slots_kernel<<<(unsigned)ceil_div(nleaf, 256), 256, 0, st>>>(nleaf, leaf_parent.p, leaf_lo.p,
+                                                                                   leaf_hi.p, nodes.p);
             LVRHIP_LAUNCH_CHECK();
+            const float4* tlo = leaf_lo.p;
+            const float4* thi = leaf_hi.p;
+            for (int j = 1, cur = 0; (1 << j) <= nleaf; j++, cur ^= 1) {
+                const int half = 1 << (j - 1);
+                range_union_level_kernel<<<(unsigned)ceil_div(nleaf - 2 * half + 1, 256), 256, 0, st>>>(
+                    nleaf, half, tlo, thi, ru_lo[cur].p, ru_hi[cur].p);
+                LVRHIP_LAUNCH_CHECK();
+                tlo = ru_lo[cur].p;
+                thi = ru_hi[cur].p;
+                refit_level_kernel<<<(unsigned)ceil_div(nint, 256), 256, 0, st>>>(nint, j, tlo, thi, nodes.p, nlo.p,
+                                                                                 nhi.p);
+                LVRHIP_LAUNCH_CHECK();
+            }
         }
         if (ns > 1) {
             scan_roots_kernel<<<(unsigned)ceil_div(ns, 256), 256, 0, st>>>(d_pstart.p, ns, leaf_parent.p, nodes.p,
