@@ -186,25 +186,32 @@ __global__ void mc_border_kernel(const uint64_t* __restrict__ keys, const uint32
     border[vals[i]] = single ? 1 : 0;
 }
 
-// cells whose own faces have one or two border edges: one move for each end point of each
+// cells whose own faces have one or two border edges: one move for each end point of each; the
+// largest count reached goes to *max_moves (one atomic per wave)
 __global__ void mc_moves_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ ntri, int64_t nc,
                                 const uint8_t* __restrict__ border, const int32_t* __restrict__ faces,
                                 uint32_t* __restrict__ moves, uint32_t* __restrict__ max_moves)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const int64_t s0 = 3 * (int64_t)off[c], s1 = s0 + 3 * (int64_t)ntri[c];
-    int cnt = 0;
-    for (int64_t s = s0; s < s1; s++) cnt += border[s];
-    if (cnt != 1 && cnt != 2) return;
-    for (int64_t s = s0; s < s1; s++) {
-        if (!border[s]) continue;
-        const int64_t f = s / 3;
-        const int b = (int)(s % 3);
-        const uint32_t u = (uint32_t)faces[3 * f + b], v = (uint32_t)faces[3 * f + (b + 1) % 3];
-        atomicMax(max_moves, atomicAdd(&moves[u], 1u) + 1u);
-        atomicMax(max_moves, atomicAdd(&moves[v], 1u) + 1u);
+    uint32_t most = 0;
+    if (c < nc) {
+        const int64_t s0 = 3 * (int64_t)off[c], s1 = s0 + 3 * (int64_t)ntri[c];
+        int cnt = 0;
+        for (int64_t s = s0; s < s1; s++) cnt += border[s];
+        if (cnt == 1 || cnt == 2) {
+            for (int64_t s = s0; s < s1; s++) {
+                if (!border[s]) continue;
+                const int64_t f = s / 3;
+                const int b = (int)(s % 3);
+                const uint32_t u = (uint32_t)faces[3 * f + b], v = (uint32_t)faces[3 * f + (b + 1) % 3];
+                most = max(most, atomicAdd(&moves[u], 1u) + 1u);
+                most = max(most, atomicAdd(&moves[v], 1u) + 1u);
+            }
+        }
     }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) most = max(most, (uint32_t)shfl_xor((int)most, m));
+    if (lane_id() == 0 && most > 0) atomicMax(max_moves, most);
 }
 
 // round r: the vertices that move at least r times, with their current positions (any order:
