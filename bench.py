@@ -11,7 +11,9 @@ A step = calculateNormals (adaptive kNN + PCA + flip) + interpolateNormals + dis
 query vertices, inputs resident in HBM.  value = points / (t_normals + t_interpolate) in
 Mpts/s; the SDF rate is reported beside it.  The roofline entry covers the dominant kernel
 (the kNN tile kernel of the normals over its adaptive-round launches), timed with hipEvents
-on the library's stream.
+on the library's stream.  The line also carries the other configurations measured in the same
+run (`forms`: the 32M-point outdoor cloud, the 64-scan project in the batched form, and the
+differentiation form; `--no-forms` skips them, `--form NAME` runs one alone).
 """
 from __future__ import annotations
 
@@ -187,7 +189,8 @@ def bench_batched(args):
 
     t_graph = timed(replay, args.steps)
     log(f"[bench] batched graph replay: {t_graph * 1e3:.1f} ms")
-    print(json.dumps({
+    b.freeGPU()
+    return {
         "metric": "batched normals Mpts/s (LBVH build + adaptive kNN/PCA + interpolation, all scans)",
         "form": "batched", "unit": "Mpts/s", "n_gpus": 1, "points": n, "scans": len(sp.sizes),
         "per_scan_handles": round(n / t_loop / 1e6, 2), "batched_eager": round(n / t_eager / 1e6, 2),
@@ -195,7 +198,7 @@ def bench_batched(args):
         "ms": {"per_scan": round(t_loop * 1e3, 1), "eager": round(t_eager * 1e3, 1), "graph": round(t_graph * 1e3, 1)},
         "autotuned": tune, "eager_bit_equal_to_per_scan": same, "eager_max_angle_to_per_scan_rad": max_angle,
         "data": "synthetic (seeded ray-cast scans of the outdoor scene)",
-    }), flush=True)
+    }
 
 
 def bench_adjoint(args):
@@ -248,14 +251,14 @@ def bench_adjoint(args):
     torch.cuda.synchronize()
     t_steps = (time.perf_counter() - t) / 20
     log(f"[bench] adjoint: forward {t_f * 1e3:.1f} ms, forward+backward {t_fb * 1e3:.1f} ms, step {t_steps * 1e3:.1f} ms")
-    print(json.dumps({
+    return {
         "metric": "SDF adjoint: forward + backward ms (1M points, 500k queries) vs forward alone",
         "form": "adjoint", "unit": "ms", "n_gpus": 1, "points": len(w.points), "queries": len(q),
         "forward_ms": round(t_f * 1e3, 2), "forward_backward_ms": round(t_fb * 1e3, 2),
         "backward_over_forward": round((t_fb - t_f) / t_f, 2), "denoise_step_ms": round(t_steps * 1e3, 2),
         "loss_first": losses[0], "loss_last": losses[-1], "steps": 20,
         "data": "synthetic (seeded noisy sphere + torus)",
-    }), flush=True)
+    }
 
 
 def bench_large(args):
@@ -293,14 +296,15 @@ def bench_large(args):
     rounds = [int(x) for x in s.normalsStats()]
     log(f"[bench] large: build {t_build:.1f} ms, normals+interp {t_norm:.1f} ms, lattice {t_lat:.1f} ms, "
         f"sdf {t_sdf:.1f} ms for {v.shape[0]} vertices")
-    print(json.dumps({
+    s.freeGPU()
+    return {
         "metric": f"large cloud normals + interpolation Mpts/s ({n} points, kn {w.kn})", "form": "large",
         "unit": "Mpts/s", "n_gpus": 1, "points": n, "value": round(n / t_norm * 1e3 / 1e6, 2),
         "sdf_mqps": round(v.shape[0] / t_sdf * 1e3 / 1e6, 2), "vertices": int(v.shape[0]),
         "ms": {"build": round(t_build, 1), "normals_interp": round(t_norm, 1), "lattice": round(t_lat, 1),
                "sdf": round(t_sdf, 1)}, "adaptive_rounds": rounds,
         "data": "synthetic (seeded ray-cast outdoor scene: Perlin terrain + 500 buildings, 200 x 200 x 30 m)",
-    }), flush=True)
+    }
 
 
 def main():
@@ -321,9 +325,12 @@ def main():
                          "large: the 32M-point cloud")
     ap.add_argument("--scans", type=int, default=64)
     ap.add_argument("--points", type=int, default=0, help="points of the large form (default 32,000,000)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="the plain line only: skip the large, batched and adjoint summaries it carries")
     args = ap.parse_args()
+    forms = {"batched": bench_batched, "adjoint": bench_adjoint, "large": bench_large}
     if args.form != "plain" and args.gpus == 1:
-        {"batched": bench_batched, "adjoint": bench_adjoint, "large": bench_large}[args.form](args)
+        print(json.dumps(forms[args.form](args)), flush=True)
         return
     if args.copy_calibration:
         import torch
@@ -525,6 +532,21 @@ def main():
     }
     if cpu:
         line["speedup_vs_cpu"] = {"normals": round(mpts / cpu["mpts"], 1), "sdf": round(sdf_mqps / cpu["sdf_mqps"], 1)}
+    if not args.no_forms:
+        # the other configurations, measured in the same run: the 32M-point outdoor cloud, the
+        # 64-scan project (batched form) and the differentiation form; a failure is recorded, not
+        # raised, so that the line above is always printed
+        s.freeGPU()
+        del pts_t, verts_t, out
+        torch.cuda.empty_cache()
+        line["forms"] = {}
+        for name in ("large", "batched", "adjoint"):
+            try:
+                log(f"[bench] form {name}")
+                line["forms"][name] = forms[name](args)
+            except Exception as e:  # noqa: BLE001 -- recorded in the line
+                line["forms"][name] = {"error": f"{type(e).__name__}: {e}"}
+            torch.cuda.empty_cache()
     print(json.dumps(line), flush=True)
 
 
