@@ -468,8 +468,8 @@ def main():
     achieved = bytes_step / launches_per_step / (avg_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
-        "limiter": "instruction issue and LDS / L2 latency: VALU busy about half the SIMD cycles at two waves "
-                   "per SIMD; measured HBM traffic is a small fraction of the algorithmic bytes",
+        "limiter": "instruction issue and LDS / L2 latency (the kernel is occupancy-bound: three waves per SIMD, "
+                   "set by LDS and registers); measured HBM traffic is a small fraction of the algorithmic bytes",
         "kernel": f"knn_tile_kernel<{lanes}, SinkNormals<{lanes}> | SinkMoments<{lanes}>> (the five adaptive-round launches)",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,

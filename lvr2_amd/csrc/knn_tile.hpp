@@ -32,14 +32,22 @@
 
 namespace lvrhip {
 
-constexpr int NB = 128;        // histogram bins
-constexpr int NB_BITS = 7;     // log2(NB): key bits resolved per refinement pass
+// LDS budget: a wave of 8 queries (P = 8) with one or two targets needs 12.4 / 13.0 KB, so that
+// twelve waves (three per SIMD) fit a CU's 160 KB; with the register budget of three waves
+// (168 VGPRs) that occupancy hides the LDS and L2 round trips of the passes better than bigger
+// chunks and finer bins at two waves per SIMD.
+constexpr int NB = 64;         // histogram bins
+constexpr int NB_BITS = 6;     // log2(NB): key bits resolved per refinement pass
 constexpr int NH = 1;          // histograms per query (lane s counts into number s % NH)
-constexpr int CAP = 512;       // candidate slots per chunk
+constexpr int CAP = 256;       // candidate slots per chunk
 constexpr int CHUNK_LEAVES = CAP / LEAF;
-constexpr int TL_CAP = 512;    // leaves kept per tile
-constexpr int STACK_CAP = 512;
-constexpr int STACK_WIDE = 256;
+constexpr int TL_CAP = 448;    // leaves kept per tile (a run's leaf list lives on the stack: <= STACK_CAP)
+constexpr int STACK_CAP = 448;
+// A wide pop takes w <= 64 entries and pushes at most 4 per entry: it needs 3 w free slots.
+constexpr int STACK_WIDE = 3 * kWave;
+static_assert(TL_CAP <= STACK_CAP, "a run's leaf list is kept on the stack");
+static_assert(STACK_CAP >= FRONTIER, "a traversal starts with up to FRONTIER subtrees on the stack");
+static_assert(CHUNK_LEAVES % (kWave / LEAF) == 0 && CHUNK_LEAVES <= kWave, "a chunk is staged in whole wave steps");
 constexpr int MAX_ATTEMPTS = 1024;    // per tile (a safety net: each cluster of queries grows x4 per
                                       // attempt after CLUSTER_GUARD of its own)
 constexpr int CLUSTER_SHRINK = 12;    // a cluster's first attempts that may shrink R or cap its growth
@@ -151,7 +159,7 @@ struct TileSmem {
     uint32_t hist[Q][NH * NB];
     int stack[STACK_CAP];
     int tleaf[TL_CAP];                    // the tile's leaves (or the current chunk's, when streaming)
-    uint8_t glist[Q][CHUNK_LEAVES + 8];   // per query: chunk leaf slots within its bound, then pads
+    uint8_t glist[Q][CHUNK_LEAVES + 4];   // per query: chunk leaf slots within its bound, then 4 pads
     uint8_t flist[Q][CHUNK_LEAVES];       // per query: chunk leaf slots taken whole (collection)
     uint64_t bkey[NT][Q][BUF];  // boundary keys (their low half, the original index, finds the point)
     uint32_t bcnt[NT][Q];
@@ -631,9 +639,17 @@ __device__ __forceinline__ void give_lds(S&, float4*, long)
 {
 }
 
+// The register budget: three waves per SIMD where the LDS allows them (P = 8 with one or two
+// targets), else up to two.
 template <int P, class Sink>
-// LDS allows about two of these waves per SIMD: the register budget is set for that occupancy.
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 2))) void knn_tile_kernel(
+constexpr bool tile_three_waves()
+{
+    return P == 8 && Sink::kTargets <= 2;
+}
+
+template <int P, class Sink>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_three_waves<P, Sink>() ? 3 : 1,
+                                                                       tile_three_waves<P, Sink>() ? 3 : 2))) void knn_tile_kernel(
     BvhView bv, QueryView qv, TileParams prm, Sink sink)
 {
     constexpr int Q = kWave / P;
