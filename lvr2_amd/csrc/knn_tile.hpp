@@ -102,6 +102,12 @@ struct TileParams {
     uint32_t* queue;            // optional work queue (QUEUE_SEGS counters QUEUE_STRIDE apart,
                                 // zeroed before the launch): waves take the next run of
                                 // tiles_per_wave tiles from it; else runs are strided over the grid
+    const float* hint_in;       // optional, per query position: the d^2 of its t-th key for an earlier
+                                // target t; a tile's first radius^2 is hint_scale times the largest
+                                // over its queries (else the radius carried from the previous tile)
+    float hint_scale;
+    float* hint_out;            // optional, per query position: the d^2 of the t0-th key of every
+                                // query resolved (may be hint_in: a query reads it before)
 };
 
 enum {
@@ -727,6 +733,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
         const int scan = (valid && bv.scan_of) ? (int)bv.scan_of[qpos] : 0;
         bool active = valid && prm.t0 > 0;
         ws.c[KS_TILES] += 1;
+        if (prm.hint_in) {
+            // the earlier target's neighbourhood of each query, grown for this one
+            const float h = uniformf(wave_max(valid ? prm.hint_in[qpos] : 0.0f));
+            if (h > 0.0f && h < 1e37f) r2 = fmaxf(h * prm.hint_scale, 1e-30f);
+        }
         if (tt == 1 && prm.tiles_per_wave > 2) {
             const auto t0 = ws.tick();
             const int64_t q0 = qbase, q1 = min((run + 1) * prm.tiles_per_wave * Q, (int64_t)nq);
@@ -1052,6 +1063,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
                     }
                 }
             }
+            if (prm.hint_out && kth >= 0.0f) prm.hint_out[qpos] = kth;
             __syncthreads();
             sink.finish(g, s, ok, qpos, qx, qy, qz, sm.scnt[0], sm.scnt[NT > 1 ? 1 : 0]);
             ws.tock(KS_CY_FINISH, t_fin);

@@ -776,6 +776,7 @@ struct lvrhip_surface {
     DBuf<uint64_t> s_k0, s_k1;
     DBuf<uint32_t> s_v0, s_v1, s_hist, s_flags, s_list0, s_list1, s_spos, s_ord, s_sample;
     DBuf<float> s_qx, s_qy, s_qz, s_f;
+    DBuf<float> s_hint;  // per sorted point: the k-th d^2 of its last adaptive round
     DBuf<int32_t> s_idx;
 
     BvhView view() const
@@ -1053,6 +1054,7 @@ struct lvrhip_surface {
         s_list1.reserve(n);
         a_sel.reserve(n);
         a_list.reserve(n);
+        s_hint.reserve(npad);
     }
 
     // calculateSurfaceNormals' adaptive rounds (AdaptiveKSearchSurface.tcc:187-284) for the
@@ -1070,6 +1072,11 @@ struct lvrhip_surface {
             TileParams prm = params(t0, r == 1 ? kiw : 0, 0);
             // the later rounds' query lists are sparse: shorter runs balance their costlier tiles
             if (r > 1) prm.tiles_per_wave = std::max(1, tiles_per_wave / 2);
+            // each round's k-th d^2 per point sizes the next round's first radius: k doubles, and
+            // on a surface the count grows like d^2
+            prm.hint_in = r > 1 ? s_hint.p : nullptr;
+            prm.hint_scale = 2.0f * r2_margin;
+            prm.hint_out = r < last ? s_hint.p : nullptr;
             prm.stats = kstats_ptr(r - 1);
             const uint32_t* qsel = r == 1 ? q0 : lists[(r - 1) & 1];
             const uint32_t* nq_dev = r == 1 ? nullptr : counter(r);
