@@ -43,8 +43,9 @@ constexpr int CAP = 256;       // candidate slots per chunk
 constexpr int CHUNK_LEAVES = CAP / LEAF;
 constexpr int TL_CAP = 448;    // leaves kept per tile (a run's leaf list lives on the stack: <= STACK_CAP)
 constexpr int STACK_CAP = 448;
-// A wide pop takes w <= 64 entries and pushes at most 4 per entry: it needs 3 w free slots.
-constexpr int STACK_WIDE = 3 * kWave;
+// A pop of w entries pushes at most 4 per entry (3 w more slots).  Wide pops keep STACK_HEADROOM
+// slots free for the depth-first pops of one entry that follow when the stack is nearly full.
+constexpr int STACK_HEADROOM = 64;
 static_assert(TL_CAP <= STACK_CAP, "a run's leaf list is kept on the stack");
 static_assert(STACK_CAP >= FRONTIER, "a traversal starts with up to FRONTIER subtrees on the stack");
 static_assert(CHUNK_LEAVES % (kWave / LEAF) == 0 && CHUNK_LEAVES <= kWave, "a chunk is staged in whole wave steps");
@@ -108,6 +109,9 @@ struct TileParams {
     float hint_scale;
     float* hint_out;            // optional, per query position: the d^2 of the t0-th key of every
                                 // query resolved (may be hint_in: a query reads it before)
+    int leaf_hint;              // the queries are the cloud's own sorted points (query position =
+                                // sorted position): a run's first tile sizes its radius from the
+                                // boxes of its queries' leaves instead of the cloud's mean density
 };
 
 enum {
@@ -205,11 +209,13 @@ __device__ __forceinline__ int fill_leaves(Sm& sm, const BvhView& bv, const Tile
     while (sp > 0) {
         const int room = cap - nl;
         if (room == 0) break;
-        // Wide pops (up to 64 entries, up to 256 pushes) while STACK_WIDE slots are free;
-        // below that, depth-first pops of one entry (up to 4 pushes), whose growth is bounded
-        // by the tree depth.  A popped node's four grandchildren are tested at once (BvhNode4).
+        // Wide pops (up to 64 entries, as many as leave STACK_HEADROOM slots free after their
+        // pushes); below that, depth-first pops of one entry (up to 4 pushes), whose growth is
+        // bounded by the tree depth.  A popped node's four grandchildren are tested at once
+        // (BvhNode4).
         const int free_slots = STACK_CAP - sp;
-        const int w = free_slots >= STACK_WIDE ? min(kWave, sp) : (free_slots >= 4 ? 1 : 0);
+        const int wide = min(min(kWave, sp), (free_slots - STACK_HEADROOM) / 3);
+        const int w = wide >= 1 ? wide : (free_slots >= 4 ? 1 : 0);
         if (w <= 0) {
             sp = -1;
             return nl;
@@ -737,6 +743,21 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
             // the earlier target's neighbourhood of each query, grown for this one
             const float h = uniformf(wave_max(valid ? prm.hint_in[qpos] : 0.0f));
             if (h > 0.0f && h < 1e37f) r2 = fmaxf(h * prm.hint_scale, 1e-30f);
+        } else if (prm.leaf_hint && tt == 0) {
+            // a leaf's 16 points on a surface patch: its box's two largest extents a, b span
+            // about (16/9) a b of surface (a 4 x 4 lattice spans 3 x 3 spacings), so t0 points
+            // lie within pi R^2 = t0 (16/9) a b / 16
+            float est = 0.0f;
+            if (valid) {
+                const uint32_t leaf = qpos / (uint32_t)LEAF;
+                const float4 lo = bv.leaf_lo[leaf], hi = bv.leaf_hi[leaf];
+                const float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
+                const float a = fmaxf(fmaxf(ex, ey), ez), c = fminf(fminf(ex, ey), ez);
+                const float b = (ex + ey + ez) - a - c;
+                est = (float)prm.t0 * a * b * (1.0f / (9.0f * 3.14159265f)) * prm.r2_margin;
+            }
+            const float h = uniformf(wave_max(est));
+            if (h > 0.0f && h < 1e37f) r2 = h;
         }
         if (tt == 1 && prm.tiles_per_wave > 2) {
             const auto t0 = ws.tick();

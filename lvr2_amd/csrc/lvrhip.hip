@@ -1077,6 +1077,7 @@ struct lvrhip_surface {
             prm.hint_in = r > 1 ? s_hint.p : nullptr;
             prm.hint_scale = 2.0f * r2_margin;
             prm.hint_out = r < last ? s_hint.p : nullptr;
+            prm.leaf_hint = 1;
             prm.stats = kstats_ptr(r - 1);
             const uint32_t* qsel = r == 1 ? q0 : lists[(r - 1) & 1];
             const uint32_t* nq_dev = r == 1 ? nullptr : counter(r);
