@@ -1073,9 +1073,10 @@ struct lvrhip_surface {
             // the later rounds' query lists are sparse: shorter runs balance their costlier tiles
             if (r > 1) prm.tiles_per_wave = std::max(1, tiles_per_wave / 2);
             // each round's k-th d^2 per point sizes the next round's first radius: k doubles, and
-            // on a surface the count grows like d^2
+            // on a surface the count grows like d^2 (x 1.15 of margin; the tuned r2_margin only
+            // sizes the radius carried from tile to tile)
             prm.hint_in = r > 1 ? s_hint.p : nullptr;
-            prm.hint_scale = 2.0f * r2_margin;
+            prm.hint_scale = 2.0f * 1.15f;
             prm.hint_out = r < last ? s_hint.p : nullptr;
             prm.leaf_hint = 1;
             prm.stats = kstats_ptr(r - 1);
