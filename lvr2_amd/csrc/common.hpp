@@ -71,17 +71,51 @@ __device__ __forceinline__ T shfl_xor(T v, int m)
     return __shfl_xor(v, m, kWave);
 }
 
+// Cross-lane reads within rows of 16 lanes by DPP (a modifier of the VALU instruction: no LDS
+// round trip, unlike a lane shuffle).  Every lane of the wave must be active.
+constexpr int DPP_QUAD_XOR1 = 0xB1;     // quad_perm [1, 0, 3, 2]: lane i <- lane i ^ 1
+constexpr int DPP_QUAD_XOR2 = 0x4E;     // quad_perm [2, 3, 0, 1]: lane i <- lane i ^ 2
+constexpr int DPP_HALF_MIRROR = 0x141;  // lane i <- lane 7 - i within each 8 lanes
+constexpr int DPP_ROW_MIRROR = 0x140;   // lane i <- lane 15 - i within each 16 lanes
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// op over each aligned group of G lanes, the result on every lane of the group.  Each step
+// combines a lane's partial with the partial of the group's other half (xor 1, xor 2, then the
+// mirrors, which pair every lane with one of the other half), so a sum adds the same two partials
+// in the same order on every lane as the xor butterfly does: the results are identical to it.
+template <int G, class Op>
+__device__ __forceinline__ float lane_group_reduce(float v, Op op)
+{
+    static_assert(G >= 1 && G <= kWave && (G & (G - 1)) == 0, "a power-of-two group");
+    if constexpr (G >= 2) v = op(v, dpp_f<DPP_QUAD_XOR1>(v));
+    if constexpr (G >= 4) v = op(v, dpp_f<DPP_QUAD_XOR2>(v));
+    if constexpr (G >= 8) v = op(v, dpp_f<DPP_HALF_MIRROR>(v));
+    if constexpr (G >= 16) v = op(v, dpp_f<DPP_ROW_MIRROR>(v));
+    if constexpr (G >= 32) v = op(v, shfl_xor(v, 16));
+    if constexpr (G >= 64) v = op(v, shfl_xor(v, 32));
+    return v;
+}
+
+__device__ __forceinline__ float readlane_f(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Minimum / maximum over the wave (every lane active), wave-uniform: per row of 16 lanes by DPP,
+// then the four rows' results read as scalars.
 __device__ __forceinline__ float wave_min(float v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = fminf(v, shfl_xor(v, m));
-    return v;
+    v = lane_group_reduce<16>(v, [](float a, float b) { return fminf(a, b); });
+    return fminf(fminf(readlane_f(v, 0), readlane_f(v, 16)), fminf(readlane_f(v, 32), readlane_f(v, 48)));
 }
 __device__ __forceinline__ float wave_max(float v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, shfl_xor(v, m));
-    return v;
+    v = lane_group_reduce<16>(v, [](float a, float b) { return fmaxf(a, b); });
+    return fmaxf(fmaxf(readlane_f(v, 0), readlane_f(v, 16)), fmaxf(readlane_f(v, 32), readlane_f(v, 48)));
 }
 
 __device__ __forceinline__ int wave_sum_int(int v)

@@ -17,26 +17,21 @@
 
 namespace lvrhip {
 
+// Sum / minimum / maximum over a query's P lanes (every lane of the wave active).
 template <int P>
 __device__ __forceinline__ float group_sum(float v)
 {
-#pragma unroll
-    for (int o = 1; o < P; o <<= 1) v += shfl_xor(v, o);
-    return v;
+    return lane_group_reduce<P>(v, [](float a, float b) { return a + b; });
 }
 template <int P>
 __device__ __forceinline__ float group_min(float v)
 {
-#pragma unroll
-    for (int o = 1; o < P; o <<= 1) v = fminf(v, shfl_xor(v, o));
-    return v;
+    return lane_group_reduce<P>(v, [](float a, float b) { return fminf(a, b); });
 }
 template <int P>
 __device__ __forceinline__ float group_max(float v)
 {
-#pragma unroll
-    for (int o = 1; o < P; o <<= 1) v = fmaxf(v, shfl_xor(v, o));
-    return v;
+    return lane_group_reduce<P>(v, [](float a, float b) { return fmaxf(a, b); });
 }
 
 // Sort the n (key, payload) pairs of one query by key (keys are unique) with the P lanes of
