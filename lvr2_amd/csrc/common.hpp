@@ -100,6 +100,47 @@ __device__ __forceinline__ float lane_group_reduce(float v, Op op)
     return v;
 }
 
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+// Integer sum over each aligned group of G <= 16 lanes (see lane_group_reduce).
+template <int G>
+__device__ __forceinline__ int lane_group_sum_int(int v)
+{
+    static_assert(G >= 1 && G <= 16 && (G & (G - 1)) == 0, "a power-of-two group within a row");
+    if constexpr (G >= 2) v += dpp_i<DPP_QUAD_XOR1>(v);
+    if constexpr (G >= 4) v += dpp_i<DPP_QUAD_XOR2>(v);
+    if constexpr (G >= 8) v += dpp_i<DPP_HALF_MIRROR>(v);
+    if constexpr (G >= 16) v += dpp_i<DPP_ROW_MIRROR>(v);
+    return v;
+}
+// Inclusive prefix sum over each aligned group of G <= 16 lanes (lane s of a group adds lanes
+// 0..s): shifts within the row, a lane taking the shifted value only from its own group.
+template <int G>
+__device__ __forceinline__ int lane_group_scan_int(int v, int s)
+{
+    static_assert(G >= 1 && G <= 16 && (G & (G - 1)) == 0, "a power-of-two group within a row");
+    if constexpr (G >= 2) {
+        const int y = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);  // row_shr:1
+        if (s >= 1) v += y;
+    }
+    if constexpr (G >= 4) {
+        const int y = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);  // row_shr:2
+        if (s >= 2) v += y;
+    }
+    if constexpr (G >= 8) {
+        const int y = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);  // row_shr:4
+        if (s >= 4) v += y;
+    }
+    if constexpr (G >= 16) {
+        const int y = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);  // row_shr:8
+        if (s >= 8) v += y;
+    }
+    return v;
+}
+
 __device__ __forceinline__ float readlane_f(float v, int lane)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));

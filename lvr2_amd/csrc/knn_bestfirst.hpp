@@ -329,7 +329,7 @@ __global__ __launch_bounds__(BF_THREADS) void bf_knn_kernel(const BvhView bv, co
                 const int seg = (xcd + seg_k) % QUEUE_SEGS;
                 uint32_t r = 0;
                 if (lane == 0) r = atomicAdd(queue + seg * QUEUE_STRIDE, 1u);
-                r = (uint32_t)uniform(shfl((int)r, 0));
+                r = (uint32_t)__builtin_amdgcn_readlane((int)r, 0);
                 const int64_t cand = ((int64_t)(r / QUEUE_CHUNK) * QUEUE_SEGS + seg) * QUEUE_CHUNK + r % QUEUE_CHUNK;
                 if (cand < nchunks) {
                     chunk = cand;

@@ -577,13 +577,8 @@ __device__ __forceinline__ int find_bin(const uint32_t (*hist)[NH * NB], int g, 
     int sum = 0;
 #pragma unroll
     for (int j = 0; j < BPL; j++) sum += hist_bin(hist, g, s * BPL + j);
-    int incl = sum;
-#pragma unroll
-    for (int o = 1; o < P; o <<= 1) {
-        int y = shfl(incl, lane_id() - o);
-        if (s >= o) incl += y;
-    }
-    const int total = shfl(incl, g * P + P - 1);
+    const int incl = lane_group_scan_int<P>(sum, s);
+    const int total = lane_group_sum_int<P>(sum);
     const int excl = incl - sum;
     bool mine = (t >= 1) && (excl < t) && (t <= incl);
     int b_ = 0, below_ = 0, cnt_ = 0;
@@ -601,12 +596,10 @@ __device__ __forceinline__ int find_bin(const uint32_t (*hist)[NH * NB], int g, 
             run += v;
         }
     }
-    const uint64_t owners = ballot(mine);
-    const uint64_t gm = (owners >> (g * P)) & ((P == 64) ? ~0ull : ((1ull << P) - 1));
-    const int owner = g * P + (gm ? __ffsll((long long)gm) - 1 : 0);
-    bin = shfl(b_, owner);
-    below = shfl(below_, owner);
-    cnt = shfl(cnt_, owner);
+    // one lane of the group at most owns the t-th key (the others hold zeros)
+    bin = lane_group_sum_int<P>(b_);
+    below = lane_group_sum_int<P>(below_);
+    cnt = lane_group_sum_int<P>(cnt_);
     return total;
 }
 
@@ -701,7 +694,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
                 const int seg = (xcd + seg_k) % QUEUE_SEGS;
                 uint32_t r = 0;
                 if (lane == 0) r = atomicAdd(prm.queue + seg * QUEUE_STRIDE, 1u);
-                r = (uint32_t)uniform(shfl((int)r, 0));
+                r = (uint32_t)__builtin_amdgcn_readlane((int)r, 0);
                 const int64_t cand = ((int64_t)(r / QUEUE_CHUNK) * QUEUE_SEGS + seg) * QUEUE_CHUNK + r % QUEUE_CHUNK;
                 if (cand < nruns) {
                     run = cand;
@@ -822,8 +815,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
                 prev_lead = lead;
             }
             cl_att++;
-            const int cur = bv.scan_of ? uniform(shfl(scan, lead)) : 0;
-            const float lx = uniformf(shfl(qx, lead)), ly = uniformf(shfl(qy, lead)), lz = uniformf(shfl(qz, lead));
+            const int cur = bv.scan_of ? __builtin_amdgcn_readlane(scan, lead) : 0;
+            const float lx = readlane_f(qx, lead), ly = readlane_f(qy, lead), lz = readlane_f(qz, lead);
             const float reach = sqrtf(r2);
             const bool near = fabsf(qx - lx) <= reach && fabsf(qy - ly) <= reach && fabsf(qz - lz) <= reach;
             const bool act = active && scan == cur && near;
@@ -1090,8 +1083,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
             ws.tock(KS_CY_FINISH, t_fin);
             // the query's nearest in-range d^2 (dmin over its lanes)
             float dm = dmin;
-#pragma unroll
-            for (int o = 1; o < P; o <<= 1) dm = fminf(dm, shfl_xor(dm, o));
+            dm = lane_group_reduce<P>(dm, [](float a, float b) { return fminf(a, b); });
             // Next tile's radius: per resolved query dmin^2 + (kth^2 - dmin^2) * margin, the
             // margin applied to the part of the radius that grows with the count (for a query
             // off a surface, R^2 - dmin^2 is what the covered disc grows with); the tile's largest.
