@@ -12,7 +12,7 @@
 //   3. gives every query its own list of the chunk's leaves whose box is within that
 //      query's current bound, and scans only those, the P lanes of the query splitting
 //      each leaf's points;
-//   4. per query, counts candidates with d^2 <= R^2 into NB bins linear in d^2.  If fewer
+//   4. per query, counts candidates with d^2 <= R^2 into bins linear in d^2.  If fewer
 //      than t candidates are in range the query stays pending and the tile retries with
 //      2R; otherwise the bin holding the t-th smallest key is found, refined by radix
 //      passes over the 64-bit key (d^2 bits, point index) until it holds at most BUF keys,
@@ -36,9 +36,12 @@ namespace lvrhip {
 // twelve waves (three per SIMD) fit a CU's 160 KB; with the register budget of three waves
 // (168 VGPRs) that occupancy hides the LDS and L2 round trips of the passes better than bigger
 // chunks and finer bins at two waves per SIMD.
-constexpr int NB = 64;         // histogram bins
-constexpr int NB_BITS = 6;     // log2(NB): key bits resolved per refinement pass
-constexpr int NH = 1;          // histograms per query (lane s counts into number s % NH)
+// Histogram per query: HIST_WORDS 32-bit words.  A pass over a tile's leaf list scans at most
+// TL_CAP leaves per query, fewer than 65,536 points, so its bins fit 16-bit counters, two to a
+// word (2 HIST_WORDS bins, HIST_BITS + 1 key bits per refinement pass; used by one-target sinks);
+// a streamed pass (no bound on its points) counts HIST_WORDS bins in whole words.
+constexpr int HIST_WORDS = 64;
+constexpr int HIST_BITS = 6;  // log2(HIST_WORDS)
 constexpr int CAP = 256;       // candidate slots per chunk
 constexpr int CHUNK_LEAVES = CAP / LEAF;
 constexpr int TL_CAP = 448;    // leaves kept per tile (a run's leaf list lives on the stack: <= STACK_CAP)
@@ -47,6 +50,7 @@ constexpr int STACK_CAP = 448;
 // slots free for the depth-first pops of one entry that follow when the stack is nearly full.
 constexpr int STACK_HEADROOM = 64;
 static_assert(TL_CAP <= STACK_CAP, "a run's leaf list is kept on the stack");
+static_assert(TL_CAP * LEAF < 65536, "16-bit bins: a list pass scans fewer than 65,536 points per query");
 static_assert(STACK_CAP >= FRONTIER, "a traversal starts with up to FRONTIER subtrees on the stack");
 static_assert(CHUNK_LEAVES % (kWave / LEAF) == 0 && CHUNK_LEAVES <= kWave, "a chunk is staged in whole wave steps");
 constexpr int MAX_ATTEMPTS = 1024;    // per tile (a safety net: each cluster of queries grows x4 per
@@ -166,7 +170,7 @@ struct TileSmem {
     // slot CHUNK_LEAVES) that pads the per-query lists to whole steps of the scan
     float4 cand[CAP + LEAF];
     float4 lbox[CHUNK_LEAVES][2];  // boxes of the chunk's leaves (lo, hi)
-    uint32_t hist[Q][NH * NB];
+    uint32_t hist[Q][HIST_WORDS];
     int stack[STACK_CAP];
     int tleaf[TL_CAP];                    // the tile's leaves (or the current chunk's, when streaming)
     uint8_t glist[Q][CHUNK_LEAVES + 4];   // per query: chunk leaf slots within its bound, then 4 pads
@@ -560,23 +564,29 @@ __device__ __forceinline__ void scan_glist(const Sm& sm, int g, int s, int len, 
 
 // Locate the bin holding the t-th smallest counted key of query group g (P lanes).
 // Returns total count; if t <= total: bin, below (count in lower bins), cnt (in that bin).
-__device__ __forceinline__ int hist_bin(const uint32_t (*hist)[NH * NB], int g, int b)
+// Bins b of a histogram in 16-bit halves (hsh = 1: word b / 2, half b % 2) or whole words (hsh = 0).
+__device__ __forceinline__ void hist_add(uint32_t* h, int b, uint32_t hsh)
 {
-    int v = 0;
-#pragma unroll
-    for (int h = 0; h < NH; h++) v += (int)hist[g][h * NB + b];
-    return v;
+    atomicAdd(&h[b >> hsh], 1u << ((b & (int)hsh) << 4));
+}
+template <bool kHalf>
+__device__ __forceinline__ int hist_bin(const uint32_t (*hist)[HIST_WORDS], int g, int b)
+{
+    if constexpr (kHalf) return (int)((hist[g][b >> 1] >> ((b & 1) << 4)) & 0xffffu);
+    return (int)hist[g][b];
 }
 
-template <int P, int Q>
-__device__ __forceinline__ int find_bin(const uint32_t (*hist)[NH * NB], int g, int s, int t, int& bin, int& below,
-                                        int& cnt)
+template <int P, int Q, int NBT>
+__device__ __forceinline__ int find_bin(const uint32_t (*hist)[HIST_WORDS], int g, int s, int t, int& bin,
+                                        int& below, int& cnt)
 {
-    constexpr int BPL = NB / P;
-    static_assert((1 << NB_BITS) == NB && NB % P == 0, "bins per lane");
+    constexpr bool kHalf = NBT == 2 * HIST_WORDS;
+    static_assert(NBT == HIST_WORDS || kHalf, "whole words or 16-bit halves");
+    constexpr int BPL = NBT / P;
+    static_assert(NBT % P == 0, "bins per lane");
     int sum = 0;
 #pragma unroll
-    for (int j = 0; j < BPL; j++) sum += hist_bin(hist, g, s * BPL + j);
+    for (int j = 0; j < BPL; j++) sum += hist_bin<kHalf>(hist, g, s * BPL + j);
     const int incl = lane_group_scan_int<P>(sum, s);
     const int total = lane_group_sum_int<P>(sum);
     const int excl = incl - sum;
@@ -587,7 +597,7 @@ __device__ __forceinline__ int find_bin(const uint32_t (*hist)[NH * NB], int g, 
 #pragma unroll
         for (int j = 0; j < BPL; j++) {
             const int b = s * BPL + j;
-            const int v = hist_bin(hist, g, b);
+            const int v = hist_bin<kHalf>(hist, g, b);
             if (run < t && t <= run + v) {
                 b_ = b;
                 below_ = run;
@@ -603,20 +613,21 @@ __device__ __forceinline__ int find_bin(const uint32_t (*hist)[NH * NB], int g, 
     return total;
 }
 
-__device__ __forceinline__ int vbin(float d2, float inv_w)
+__device__ __forceinline__ int vbin(float d2, float inv_w, int nb)
 {
     float f = d2 * inv_w;
     int b = (int)f;
-    return b < NB - 1 ? b : NB - 1;
+    return b < nb - 1 ? b : nb - 1;
 }
 
-// Smallest non-negative float x with vbin(x, inv_w) >= b (b >= 1).
-__device__ __forceinline__ float vbin_floor(int b, float inv_w)
+// Smallest non-negative float x with vbin(x, inv_w, nb) >= b (b >= 1).
+__device__ __forceinline__ float vbin_floor(int b, float inv_w, int nb)
 {
     float x = (float)b / inv_w;
     if (!(x >= 0.0f) || isinf(x)) x = 0.0f;
-    for (int it = 0; it < 256 && x > 0.0f && vbin(bitsf(fbits(x) - 1u), inv_w) >= b; it++) x = bitsf(fbits(x) - 1u);
-    for (int it = 0; it < 256 && vbin(x, inv_w) < b; it++) x = bitsf(fbits(x) + 1u);
+    for (int it = 0; it < 256 && x > 0.0f && vbin(bitsf(fbits(x) - 1u), inv_w, nb) >= b; it++)
+        x = bitsf(fbits(x) - 1u);
+    for (int it = 0; it < 256 && vbin(x, inv_w, nb) < b; it++) x = bitsf(fbits(x) + 1u);
     return x;
 }
 
@@ -829,7 +840,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
             tb.hy = uniformf(wave_max(act ? qy : -INFINITY));
             tb.hz = uniformf(wave_max(act ? qz : -INFINITY));
             const float R2 = r2;
-            const float inv_w = (float)NB / R2;
             ws.c[KS_ATTEMPTS] += 1;
 
             // --- candidate leaves of the tile -------------------------------------------------
@@ -883,9 +893,23 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
             }
 
             // --- pass 1: value histogram over [0, R2] ---------------------------------------
-            for (int b = s; b < NH * NB; b += P) sm.hist[g][b] = 0u;
+            // 16-bit bins over a leaf list, whole words when streaming.  One-target sinks only: their
+            // large k puts more keys in a bin (the finer bins spare refinement passes); with a ki
+            // target the wider bin search cost more than it saved.
+            const bool narrow = NT == 1 && cs.ntl >= 0;
+            const uint32_t hsh = narrow ? 1u : 0u;
+            const int nb = HIST_WORDS << hsh;
+            const int nbb = HIST_BITS + (int)hsh;
+            const float inv_w = (float)nb / R2;
+            auto locate = [&](int t, int& b, int& below, int& cnt) {
+                if constexpr (NT == 1) {
+                    if (narrow) return find_bin<P, Q, 2 * HIST_WORDS>(sm.hist, g, s, t, b, below, cnt);
+                }
+                return find_bin<P, Q, HIST_WORDS>(sm.hist, g, s, t, b, below, cnt);
+            };
+            for (int b = s; b < HIST_WORDS; b += P) sm.hist[g][b] = 0u;
             __syncthreads();
-            uint32_t* myhist = &sm.hist[g][(s % NH) * NB];
+            uint32_t* myhist = &sm.hist[g][0];
             float dmin = INFINITY;  // nearest in-range candidate of this lane
             bool fine = for_each_chunk(sm, bv, tb, R2, cur, cs, ws, [&](int nl, int base) {
                 auto t0 = ws.tick();
@@ -895,7 +919,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
                 scan_glist<P>(sm, g, s, len, [&](const float4& p, int) {
                     const float d2 = d2_contract(qx, qy, qz, p.x, p.y, p.z);
                     if (d2 <= R2) {
-                        atomicAdd(&myhist[vbin(d2, inv_w)], 1u);
+                        hist_add(myhist, vbin(d2, inv_w, nb), hsh);
                         dmin = fminf(dmin, d2);
                     }
                 });
@@ -915,7 +939,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
             int in_range = 0;
             {
                 int b, below, cnt;
-                const int total = find_bin<P, Q>(sm.hist, g, s, prm.t0, b, below, cnt);
+                const int total = locate(prm.t0, b, below, cnt);
                 in_range = total;
                 ok = act && total >= prm.t0;
 #pragma unroll
@@ -928,15 +952,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
                     tg[tau].klo = ~0ull;
                     tg[tau].khi = ~0ull;
                     if (tau >= 1) {
-                        if (t > 0) find_bin<P, Q>(sm.hist, g, s, t, b, below, cnt);
+                        if (t > 0) locate(t, b, below, cnt);
                     }
                     if (ok && t > 0) {
-                        const float lo = b == 0 ? 0.0f : vbin_floor(b, inv_w);
+                        const float lo = b == 0 ? 0.0f : vbin_floor(b, inv_w, nb);
                         tg[tau].klo = (uint64_t)fbits(lo) << 32;
-                        if (b == NB - 1 || !(inv_w > 0.0f)) {
+                        if (b == nb - 1 || !(inv_w > 0.0f)) {
                             tg[tau].khi = ((uint64_t)fbits(R2) << 32) | 0xffffffffull;
                         } else {
-                            const float hi = vbin_floor(b + 1, inv_w);
+                            const float hi = vbin_floor(b + 1, inv_w, nb);
                             tg[tau].khi = ((uint64_t)fbits(hi) << 32) - 1ull;
                         }
                         tg[tau].below = below;
@@ -955,10 +979,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
                     ws.c[KS_REFINE] += 1;
                     const uint64_t span = tg[tau].khi - tg[tau].klo;
                     const int nbits = span ? 64 - __clzll((long long)span) : 0;
-                    const int shift = nbits > NB_BITS ? nbits - NB_BITS : 0;
+                    const int shift = nbits > nbb ? nbits - nbb : 0;
                     const float rq = need ? bitsf((uint32_t)(tg[tau].khi >> 32)) : -1.0f;
                     const float rin = need ? bitsf((uint32_t)(tg[tau].klo >> 32)) : -1.0f;
-                    for (int b = s; b < NH * NB; b += P) sm.hist[g][b] = 0u;
+                    for (int b = s; b < HIST_WORDS; b += P) sm.hist[g][b] = 0u;
                     __syncthreads();
                     // a streamed pass walks the tree only as far as the widest refined bin reaches
                     const float rq_all = fminf(uniformf(wave_max(rq)), R2);
@@ -970,7 +994,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
                             if (!(d2 <= rq && d2 >= rin)) return;
                             const uint64_t key = make_key(d2, __float_as_uint(p.w));
                             if (key >= tg[tau].klo && key <= tg[tau].khi)
-                                atomicAdd(&myhist[(int)((key - tg[tau].klo) >> shift)], 1u);
+                                hist_add(myhist, (int)((key - tg[tau].klo) >> shift), hsh);
                         });
                     });
                     if (!fine) {
@@ -978,7 +1002,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(tile_thre
                         return;
                     }
                     int b, below, cnt;
-                    find_bin<P, Q>(sm.hist, g, s, tg[tau].t - tg[tau].below, b, below, cnt);
+                    locate(tg[tau].t - tg[tau].below, b, below, cnt);
                     if (need) {
                         const uint64_t nlo = tg[tau].klo + ((uint64_t)b << shift);
                         uint64_t nhi = nlo + ((shift >= 64) ? ~0ull : ((1ull << shift) - 1ull));
