@@ -80,6 +80,33 @@ def test_identical_points(oracle_mod):
     np.testing.assert_array_equal(ge, oe)
 
 
+def test_tied_piles_streamed(oracle_mod):
+    """Ten points at x = 0 next to two piles of 10,000 copies at x = 1 and x = 2 (collinear): the
+    ten points' k-sets (k = 20 ... 320) take the nearest copies at x = 1 by index, their boxes
+    fail every round, and each search's tile list (the 625 leaves of a pile, all at one d^2)
+    outgrows a tile's 448 leaves, so the one-target rounds stream and count whole-word bins (a
+    pile's own points pass at k = 20, their boxes being empty)."""
+    rng = np.random.default_rng(5)
+    p = np.concatenate([np.zeros((10, 3)), np.tile([[1.0, 0.0, 0.0]], (10_000, 1)),
+                        np.tile([[2.0, 0.0, 0.0]], (10_000, 1))]).astype(np.float32)
+    p = p[rng.permutation(len(p))]
+    o = oracle_mod.Surface(p)
+    s = _surface(p)
+    lone = np.flatnonzero(p[:, 0] == 0.0)
+    for k in (64, 320):
+        gi, gd = s.kSearch(p[lone], k)
+        oi, od = o.knn(p[lone], k)
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gd, od)
+    s.setKn(10)
+    s.setKi(10)
+    s.calculateNormals()
+    ak = s.adaptiveK()
+    np.testing.assert_array_equal(ak, o.normals(10, 3, (0.0, 0.0, 0.0))[2])
+    assert np.all(ak[lone] == 320) and np.all(np.delete(ak, lone) == 20)
+    assert np.all(np.isfinite(s.getNormals()))
+
+
 def test_duplicated_positions(oracle_mod):
     """1,000 positions, each 20 times, shuffled: ties broken by index in every search."""
     rng = np.random.default_rng(3)
