@@ -536,9 +536,9 @@ __device__ __forceinline__ void scan_glist(const Sm& sm, int g, int s, int len, 
     constexpr int U = LEAF / P;  // points per lane per leaf
     constexpr int L = U >= 4 ? 1 : 4 / U;  // leaves per step: at least 4 LDS reads in flight
     static_assert(LEAF % P == 0, "lanes per query must divide the leaf size");
-    static_assert(L <= 4, "the lists carry 4 pads");
-    // the next step's list entries are read one step ahead (the rows hold 8 entries past
-    // their last leaf: 4 pads, then up to L - 1 entries that are read but never used)
+    // the next step's list entries are read one step ahead: a step reads up to 2 L - 2 entries past
+    // the row's last leaf, which the row's 4 pads (the NaN leaf) cover
+    static_assert(2 * L - 2 <= 4, "the lists carry 4 pads");
     int nxt[L];
 #pragma unroll
     for (int l = 0; l < L; l++) nxt[l] = sm.glist[g][l];
