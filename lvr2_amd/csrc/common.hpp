@@ -77,6 +77,7 @@ constexpr int DPP_QUAD_XOR1 = 0xB1;     // quad_perm [1, 0, 3, 2]: lane i <- lan
 constexpr int DPP_QUAD_XOR2 = 0x4E;     // quad_perm [2, 3, 0, 1]: lane i <- lane i ^ 2
 constexpr int DPP_HALF_MIRROR = 0x141;  // lane i <- lane 7 - i within each 8 lanes
 constexpr int DPP_ROW_MIRROR = 0x140;   // lane i <- lane 15 - i within each 16 lanes
+constexpr int DPP_ROW_SHR = 0x110;      // + n (1..15): lane i <- lane i - n within each 16 lanes
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v)
 {
@@ -117,25 +118,26 @@ __device__ __forceinline__ int lane_group_sum_int(int v)
     return v;
 }
 // Inclusive prefix sum over each aligned group of G <= 16 lanes (lane s of a group adds lanes
-// 0..s): shifts within the row, a lane taking the shifted value only from its own group.
+// 0..s): shifts within the row (a lane shifted in from outside the row reads 0), a lane taking
+// the shifted value only from its own group.
 template <int G>
 __device__ __forceinline__ int lane_group_scan_int(int v, int s)
 {
     static_assert(G >= 1 && G <= 16 && (G & (G - 1)) == 0, "a power-of-two group within a row");
     if constexpr (G >= 2) {
-        const int y = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);  // row_shr:1
+        const int y = __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 1, 0xF, 0xF, true);
         if (s >= 1) v += y;
     }
     if constexpr (G >= 4) {
-        const int y = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);  // row_shr:2
+        const int y = __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 2, 0xF, 0xF, true);
         if (s >= 2) v += y;
     }
     if constexpr (G >= 8) {
-        const int y = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);  // row_shr:4
+        const int y = __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 4, 0xF, 0xF, true);
         if (s >= 4) v += y;
     }
     if constexpr (G >= 16) {
-        const int y = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);  // row_shr:8
+        const int y = __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 8, 0xF, 0xF, true);
         if (s >= 8) v += y;
     }
     return v;
