@@ -171,8 +171,9 @@ int lvrhip_interpolated_normals(lvrhip_t h, const float* q, size_t m, float* out
 
 /* Search algorithm of the searches with at most 32 neighbours (distances, kSearch, radius
  * search, interpolated normals at positions); no lvr2 counterpart.  LVRHIP_SEARCH_AUTO (the
- * default) runs them best-first, one lane per query; the queries whose traversal stack
- * overflows are searched again by one wave each, and any that outgrow that by the tile kernel.
+ * default) runs them best-first, one lane per query (a batch of at most 1,024 queries one wave
+ * per query); the queries whose traversal stack overflows are searched again by one wave each,
+ * and any that outgrow that by the tile kernel.
  * LVRHIP_SEARCH_TILE forces the tile kernel, LVRHIP_SEARCH_WAVE the wave-per-query search for
  * every query.  All give identical results. */
 enum { LVRHIP_SEARCH_AUTO = 0, LVRHIP_SEARCH_TILE = 1, LVRHIP_SEARCH_WAVE = 2 };

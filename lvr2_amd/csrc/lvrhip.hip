@@ -2026,7 +2026,9 @@ struct lvrhip_surface {
     {
         s_redo.reserve(m);
         s_redo_list.reserve(m);
-        if (search_mode == LVRHIP_SEARCH_WAVE) {
+        // A small batch goes to the wave-per-query search at once: the lane search would run it on
+        // a few waves, each as slow as its slowest lane, while every query here gets a wave.
+        if (search_mode == LVRHIP_SEARCH_WAVE || (search_mode == LVRHIP_SEARCH_AUTO && m <= BF_WAVE_BLOCKS)) {
             fill_u32_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(s_redo.p, m, 1u);
             LVRHIP_LAUNCH_CHECK();
         } else {
