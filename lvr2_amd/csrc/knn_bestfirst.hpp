@@ -91,6 +91,13 @@ struct BfList {
     __device__ __forceinline__ bool full() const { return sp[K - 1] != ~0u; }
     // d^2 of the t-th key: the pruning bound
     __device__ __forceinline__ float bound() const { return bitsf((uint32_t)(key[K - 1] >> 32)); }
+    __device__ __forceinline__ bool contains(uint64_t c) const
+    {
+        bool f = false;
+#pragma unroll
+        for (int j = 0; j < K; j++) f = f || key[j] == c;
+        return f;
+    }
 
     // insert c (c < worst()), dropping the largest
     __device__ __forceinline__ void insert(uint64_t c, uint32_t s)
@@ -117,7 +124,9 @@ struct BfList {
 // The leaf's points whose key beats the current t-th enter the list smallest first: each round
 // finds the smallest remaining candidate (compile-time indices only) and inserts it, so a wave
 // runs as many insertion rounds as its busiest lane has improving points, not one per point.
-template <int K>
+// kDedup: the list may already hold some of the leaf's points (a search started from a known
+// neighbour set); those are not inserted twice.
+template <int K, bool kDedup = false>
 __device__ __forceinline__ void bf_scan_leaf(const BvhView& bv, int leaf, float qx, float qy, float qz, BfList<K>& L)
 {
     const float4* p = bv.pts4 + (size_t)leaf * LEAF;
@@ -144,7 +153,7 @@ __device__ __forceinline__ void bf_scan_leaf(const BvhView& bv, int leaf, float 
             }
         }
         if (!(best < L.worst())) break;  // the rest cannot enter (~0 marks none)
-        L.insert(best, (uint32_t)(leaf * LEAF + bj));
+        if (!kDedup || !L.contains(best)) L.insert(best, (uint32_t)(leaf * LEAF + bj));
 #pragma unroll
         for (int j = 0; j < LEAF; j++)
             if (j == bj) c[j] = ~0ull;
@@ -156,6 +165,15 @@ __device__ __forceinline__ float bf_child_d2(const float* f, int k, float qx, fl
     return box_d2(f[6 * k], f[6 * k + 1], f[6 * k + 2], f[6 * k + 3], f[6 * k + 4], f[6 * k + 5], qx, qy, qz, qx, qy,
                   qz);
 }
+
+// Outputs that start a query's search from a known neighbour set declare kPrefill = true and
+// prefill_list(i, L, t, x, y, z).
+template <class O, class = void>
+struct bf_prefills : std::false_type {
+};
+template <class O>
+struct bf_prefills<O, std::void_t<decltype(O::kPrefill)>> : std::integral_constant<bool, O::kPrefill> {
+};
 
 // stats (optional, KS_* counters of knn_tile.hpp): waves (KS_TILES), the largest leaf count of a
 // lane summed over waves (KS_ATTEMPTS: the busiest lane's leaves), points scanned
@@ -232,6 +250,7 @@ __global__ __launch_bounds__(BF_THREADS) void bf_knn_kernel(const BvhView bv, co
                 y = qy[gi * qs];
                 z = qz[gi * qs];
                 L.init(t, empty);
+                if constexpr (bf_prefills<Out>::value) out.template prefill_list<K>(gi, L, t, x, y, z);
             }
             want = ballot(!busy);
             if (want != 0) seg_k++;  // the segment ran out (its indices only grow)
@@ -346,7 +365,7 @@ __global__ __launch_bounds__(BF_THREADS) void bf_knn_kernel(const BvhView bv, co
         }
         // 3. the leaves, scanned together
         if (leaf >= 0) {
-            bf_scan_leaf<K>(bv, leaf, x, y, z, L);
+            bf_scan_leaf<K, bf_prefills<Out>::value>(bv, leaf, x, y, z, L);
             leaves++;
         }
         // 4. finished queries
@@ -733,6 +752,30 @@ struct BfCentroidOut {
     const uint32_t* qorig;
     float* out;  // caller order, 3 floats per query
     uint32_t* redo;
+    // optional: the query's vertex (caller index -> vertex id) and, per vertex, the t neighbours
+    // (sorted positions) of its last search, which a later search of the vertex starts from
+    // (prefill): their keys at the new position bound the search at once.  The result is the
+    // same exact neighbour set.
+    const uint32_t* vid;
+    uint32_t* nbr;
+    bool prefill;
+    uint32_t npts;  // sorted positions
+    static constexpr bool kPrefill = true;
+
+    template <int K>
+    __device__ __forceinline__ void prefill_list(int64_t i, BfList<K>& L, int t, float qx, float qy, float qz) const
+    {
+        if (!prefill || !vid || !nbr) return;
+        const size_t v = vid[qorig[i]];
+        for (int j = 0; j < t; j++) {
+            const uint32_t s = nbr[v * t + j];
+            if (s >= npts) continue;
+            const float4 p = pts[s];
+            const float d = d2_contract(qx, qy, qz, p.x, p.y, p.z);
+            const uint64_t key = make_key(d, fbits(p.w));
+            if (d == d && key < L.worst() && !L.contains(key)) L.insert(key, s);
+        }
+    }
 
     template <int K>
     __device__ __forceinline__ void emit(int64_t i, const BfList<K>& L, int t, bool ok, float qx, float qy,
@@ -740,6 +783,12 @@ struct BfCentroidOut {
     {
         redo[i] = ok ? 0u : 1u;
         if (!ok) return;
+        if (vid && nbr) {
+            const size_t v = vid[qorig[i]];
+#pragma unroll
+            for (int j = 0; j < K; j++)  // compile-time slots: the list stays in registers
+                if (j >= K - t) nbr[v * t + (j - (K - t))] = L.sp[j];
+        }
         float cx = 0.0f, cy = 0.0f, cz = 0.0f;
         int nk = 0;
 #pragma unroll

@@ -1795,6 +1795,7 @@ struct lvrhip_surface {
     // ---- marching cubes over the lattice (mesh.hpp) -------------------------------------------
     DBuf<uint8_t> mc_case, mc_keep, mc_border;
     DBuf<uint32_t> mc_ntri, mc_off, mc_sums, mc_fcell, mc_vals, mc_moves, mc_list;
+    DBuf<uint32_t> mc_nbr;  // per mesh vertex: the neighbours of its last centroid search
     DBuf<uint64_t> mc_keys;
     DBuf<float> mesh_vtx, mc_q, mc_c;
     DBuf<int32_t> mesh_faces;
@@ -1905,6 +1906,7 @@ struct lvrhip_surface {
                                                                     mesh_faces.p, mc_moves.p, counter(11));
         LVRHIP_LAUNCH_CHECK();
         const uint32_t rounds = read_counter(11);
+        mc_nbr.reserve((size_t)nv * kc);
         for (uint32_t r = 1; r <= rounds; r++) {
             LVRHIP_CHECK(hipMemsetAsync(counter(12), 0, sizeof(uint32_t), st));
             mc_list.reserve(nv);
@@ -1915,7 +1917,8 @@ struct lvrhip_surface {
             const int64_t mr = read_counter(12);
             if (mr == 0) break;
             mc_c.reserve(3 * mr);
-            centroids_device(mc_q.p, mr, kc, mc_c.p);
+            // a vertex's later rounds start from its previous round's neighbours
+            centroids_device(mc_q.p, mr, kc, mc_c.p, mc_list.p, mc_nbr.p, r > 1);
             mc_round_scatter_kernel<<<(unsigned)ceil_div(mr, 256), 256, 0, st>>>(mc_list.p, mr, mc_c.p, mesh_vtx.p);
             LVRHIP_LAUNCH_CHECK();
             mesh_moves += mr;
@@ -2118,7 +2121,10 @@ struct lvrhip_surface {
 
     // The centroid of the kc cloud points nearest to each of m device positions (the vertex move
     // of BilinearFastBox::optimizePlanarFaces, BilinearFastBox.tcc:168-204); d_out in caller order.
-    void centroids_device(const float* d_q, int64_t m, int kc, float* d_out)
+    // vid / nbr (optional): each query's mesh vertex and, per vertex, the neighbours of its last
+    // search, saved here and, with prefill, the start of this search.
+    void centroids_device(const float* d_q, int64_t m, int kc, float* d_out, const uint32_t* vid = nullptr,
+                          uint32_t* nbr = nullptr, bool prefill = false)
     {
         require_one_scan("the mesh");
         if (m == 0) return;
@@ -2127,7 +2133,7 @@ struct lvrhip_surface {
         if (best_first(t0)) {
             sort_queries(d_q, m);
             s_redo.reserve(m);
-            BfCentroidOut o{pts4.p, s_v0.p, d_out, s_redo.p};
+            BfCentroidOut o{pts4.p, s_v0.p, d_out, s_redo.p, vid, nbr, prefill, (uint32_t)npad};
             enqueue_bf(m, t0, BF_EMPTY, o, 7);
             if (bf_redo_count() == 0) {
                 check_err();
