@@ -430,9 +430,8 @@ def main():
     if ks["tiles"]:
         log(f"[bench] SDF search: {ks['resolved'] / args.steps:.0f} vertices one lane each, "
             f"{ks['refine'] / args.steps:.0f} left to the wave pass; per vertex {ks['candidates'] / max(ks['resolved'], 1):.1f} "
-            f"points scanned, {ks['pops'] / max(ks['resolved'], 1):.1f} nodes expanded; busiest lane of a wave "
-            f"{16 * ks['attempts'] / ks['tiles']:.0f} points, {ks['resolved'] / ks['tiles']:.0f} vertices and "
-            f"{ks['cy_total'] / ks['tiles']:.0f} cycles per wave")
+            f"points scanned, {ks['pops'] / max(ks['resolved'], 1):.1f} nodes expanded; slowest lane per wave "
+            f"{16 * ks['attempts'] / ks['tiles']:.1f} points; {ks['cy_total'] / ks['tiles']:.0f} cycles per wave")
     if kw["tiles"]:
         log(f"[bench] SDF wave pass: {kw['resolved'] / args.steps:.0f} vertices, {kw['refine'] / args.steps:.0f} left "
             f"to the tile kernel; per vertex {kw['candidates'] / kw['tiles']:.0f} points, {kw['pops'] / kw['tiles']:.0f} "

@@ -166,6 +166,114 @@ __device__ __forceinline__ float bf_child_d2(const float* f, int k, float qx, fl
                   qz);
 }
 
+// The search of one lane; returns false if its stack overflowed (the list is then incomplete).
+template <int K, bool kDedup, class Stack>
+__device__ __forceinline__ bool bf_search(const BvhView& bv, const Stack& stk, float qx, float qy, float qz,
+                                          BfList<K>& L, int& leaves, int& nodes)
+{
+    const int root = bv.nleaf > 1 ? 0 : leaf_ref(0);
+    // 1. greedy descent: the seed leaf
+    int ref = root;
+    while (ref >= 0) {
+        const BvhNode4 nd = bv.nodes4[ref];
+        const float* f = reinterpret_cast<const float*>(nd.b);
+        const int rr[4] = {nd.r.x, nd.r.y, nd.r.z, nd.r.w};
+        float best = INFINITY;
+        int next = rr[0];  // slot 0 always holds a child
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float d = bf_child_d2(f, k, qx, qy, qz);
+            if (rr[k] != EMPTY4 && d < best) {
+                best = d;
+                next = rr[k];
+            }
+        }
+        ref = next;
+    }
+    const int seed = -1 - ref;
+    bf_scan_leaf<K, kDedup>(bv, seed, qx, qy, qz, L);
+    leaves++;
+    if (root < 0) return true;
+    // 2. best-first from the root.  Each pass pops (expanding nodes) until the lane holds a leaf
+    // to scan, then the lanes scan their leaves together: leaf scans, the costly part, run with
+    // most lanes of the wave active instead of interleaving with other lanes' node expansions.
+    int sp = 0;
+    uint2 e = make_uint2((uint32_t)root, 0u);  // the entry to process next
+    bool have = true;                           // e holds an entry (else pop one)
+    for (;;) {
+        int leaf = -1;
+        for (;;) {
+            if (!have) {
+                if (sp == 0) break;
+                e = stk.get(--sp);
+            }
+            have = false;
+            const float bound = L.bound();
+            if (bitsf(e.y) > bound) continue;
+            const int r = (int)e.x;
+            if (r < 0) {
+                if (-1 - r != seed) {
+                    leaf = -1 - r;
+                    break;
+                }
+                continue;
+            }
+            nodes++;
+            const BvhNode4 nd = bv.nodes4[r];
+            const float* f = reinterpret_cast<const float*>(nd.b);
+            int c[4] = {nd.r.x, nd.r.y, nd.r.z, nd.r.w};
+            float d[4];
+            int nv = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                d[k] = bf_child_d2(f, k, qx, qy, qz);
+                if (c[k] == EMPTY4 || !(d[k] <= bound)) {
+                    c[k] = EMPTY4;
+                    d[k] = -1.0f;  // sorts last
+                } else {
+                    nv++;
+                }
+            }
+            // descending by distance: the valid children first, the nearest of them last
+            auto cswap = [&](int x, int y) {
+                if (d[x] < d[y]) {
+                    const float td = d[x];
+                    d[x] = d[y];
+                    d[y] = td;
+                    const int tc = c[x];
+                    c[x] = c[y];
+                    c[y] = tc;
+                }
+            };
+            cswap(0, 1);
+            cswap(2, 3);
+            cswap(0, 2);
+            cswap(1, 3);
+            cswap(1, 2);
+            // the nearest child is processed next without a trip through the stack; the others
+            // are pushed farthest first
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (k + 1 < nv) {
+                    if (sp == stk.capacity()) return false;
+                    stk.put(sp, make_uint2((uint32_t)c[k], fbits(d[k])));
+                    sp++;
+                } else if (k + 1 == nv) {
+                    e = make_uint2((uint32_t)c[k], fbits(d[k]));
+                    have = true;
+                }
+            }
+        }
+        if (leaf < 0) break;  // stack empty: the list is final
+        bf_scan_leaf<K, kDedup>(bv, leaf, qx, qy, qz, L);
+        leaves++;
+    }
+    return true;
+}
+
+// One lane per query i < m (queries in Morton order for locality): Out(i, list, t, ok, q)
+// consumes the result.  empty: the key an unfilled slot holds (BF_EMPTY, or (r^2, 0) to keep
+// only points with d^2 < r^2).
 // Outputs that start a query's search from a known neighbour set declare kPrefill = true and
 // prefill_list(i, L, t, x, y, z).
 template <class O, class = void>
@@ -176,21 +284,44 @@ struct bf_prefills<O, std::void_t<decltype(O::kPrefill)>> : std::integral_consta
 };
 
 // stats (optional, KS_* counters of knn_tile.hpp): waves (KS_TILES), the largest leaf count of a
-// lane summed over waves (KS_ATTEMPTS: the busiest lane's leaves), points scanned
+// lane summed over waves (KS_ATTEMPTS: a wave runs as long as its slowest lane), points scanned
+// (KS_CANDIDATES), nodes expanded (KS_POPS), queries left to the tile kernel (KS_REFINE),
+// queries resolved (KS_RESOLVED), shader-clock cycles per wave (KS_CY_TOTAL).
+template <int K, class Out>
+__device__ __forceinline__ void bf_stats(unsigned long long* stats, bool valid, bool ok, int leaves, int nodes,
+                                         unsigned long long c0)
+{
+    int lmax = leaves, lsum = leaves, nsum = nodes, bad = (valid && !ok) ? 1 : 0, good = (valid && ok) ? 1 : 0;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        lmax = max(lmax, shfl_xor(lmax, o));
+        lsum += shfl_xor(lsum, o);
+        nsum += shfl_xor(nsum, o);
+        bad += shfl_xor(bad, o);
+        good += shfl_xor(good, o);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&stats[KS_TILES], 1ull);
+        atomicAdd(&stats[KS_ATTEMPTS], (unsigned long long)lmax);
+        atomicAdd(&stats[KS_CANDIDATES], (unsigned long long)lsum * LEAF);
+        atomicAdd(&stats[KS_POPS], (unsigned long long)nsum);
+        atomicAdd(&stats[KS_REFINE], (unsigned long long)bad);
+        atomicAdd(&stats[KS_RESOLVED], (unsigned long long)good);
+        atomicAdd(&stats[KS_CY_TOTAL], cycles() - c0);
+    }
+}
+
+// stats (optional, KS_* counters of knn_tile.hpp): waves (KS_TILES), the largest leaf count of a
+// lane summed over waves (KS_ATTEMPTS: a wave runs as long as its slowest lane), points scanned
 // (KS_CANDIDATES), nodes expanded (KS_POPS), queries left to the next pass (KS_REFINE),
 // queries resolved (KS_RESOLVED), shader-clock cycles per wave (KS_CY_TOTAL).
 //
-// One lane per query (queries in Morton order).  A lane that finishes its query takes the next
-// one at once instead of waiting for the wave's slowest lane: each step, the lanes without a
-// query take consecutive queries with one atomic per wave, every lane advances its search until
-// it holds a leaf to scan (a new query descends greedily to its seed leaf; the others pop the
-// best-first stack), then the lanes scan their leaves together, and the lanes whose stack ran
-// empty emit their result.  With a work queue (QUEUE_SEGS counters QUEUE_STRIDE apart, zeroed
-// before the launch; the grid is then the waves the device holds at once) the queries are dealt
-// in blocks of QUEUE_CHUNK * 64 consecutive queries to the XCDs in turn, each XCD taking from its
-// own counter and then helping the others, so neighbouring queries share an L2 and the costly
-// regions of a cloud are spread over all XCDs.  Without a queue, wave b takes queries
-// [64 b, 64 b + 64) once.
+// One lane per query (queries in Morton order), chunks of 64 consecutive queries per wave.  With
+// a work queue (QUEUE_SEGS counters QUEUE_STRIDE apart, zeroed before the launch; the grid is
+// then the waves the device holds at once) the chunks are dealt like the tile kernel's runs:
+// blocks of QUEUE_CHUNK consecutive chunks to the XCDs in turn, each XCD taking from its own
+// counter and then helping the others, so neighbouring chunks share an L2 and the costly regions
+// of a cloud are spread over all XCDs.  Without a queue, wave b takes chunk b.
 //
 // Query i is (qx[i qs], qy[i qs], qz[i qs]): qs = 1 for SoA coordinates, 3 for an AoS buffer
 // (qy = qx + 1, qz = qx + 2).  spill: spill_cap entries per lane for each wave of the grid.
@@ -202,199 +333,49 @@ __global__ __launch_bounds__(BF_THREADS) void bf_knn_kernel(const BvhView bv, co
                                                             unsigned long long* stats, uint32_t* queue, uint2* spill,
                                                             int spill_cap)
 {
-    __shared__ uint2 stk_lds[BF_STACK][BF_THREADS];
+    __shared__ uint2 stk[BF_STACK][BF_THREADS];
     uint2* const my_spill = spill + (size_t)blockIdx.x * spill_cap * BF_THREADS + threadIdx.x;
-    const BfLdsStack stk{(lds_uint2*)&stk_lds[0][threadIdx.x], my_spill, BF_STACK + spill_cap};
     const int lane = threadIdx.x;
-    const int root = bv.nleaf > 1 ? 0 : leaf_ref(0);
-    constexpr int64_t BLOCK = (int64_t)QUEUE_CHUNK * BF_THREADS;  // queries dealt to a segment at a time
+    const int64_t nchunks = (m + BF_THREADS - 1) / BF_THREADS;
     const int xcd = (int)(blockIdx.x % QUEUE_SEGS);
     int seg_k = 0;
-    bool dealt = false;  // without a queue: the wave's one block of queries taken
-    const unsigned long long c0 = stats ? cycles() : 0ull;
-    int leaves = 0, nodes = 0, n_ok = 0, n_bad = 0;
-
-    // the lane's query and search state
-    bool busy = false, descend = false, ok = true;
-    int64_t qi = 0;
-    float x = 0.0f, y = 0.0f, z = 0.0f;
-    BfList<K> L;
-    L.init(t, empty);
-    int sp = 0, seed = -1;
-    uint2 e = make_uint2(0u, 0u);
-    bool have = false;
+    bool once = false;
     for (;;) {
-        // 1. lanes without a query take the next ones
-        uint64_t want = ballot(!busy);
-        while (want != 0) {
-            int64_t gi = -1;
-            if (queue) {
-                if (seg_k >= QUEUE_SEGS) break;
+        int64_t chunk = -1;
+        if (queue) {
+            while (seg_k < QUEUE_SEGS) {
                 const int seg = (xcd + seg_k) % QUEUE_SEGS;
                 uint32_t r = 0;
-                if (lane == 0) r = atomicAdd(queue + seg * QUEUE_STRIDE, (uint32_t)__popcll(want));
+                if (lane == 0) r = atomicAdd(queue + seg * QUEUE_STRIDE, 1u);
                 r = (uint32_t)__builtin_amdgcn_readlane((int)r, 0);
-                const uint64_t rr = (uint64_t)r + (uint64_t)mbcnt(want);
-                gi = ((int64_t)(rr / BLOCK) * QUEUE_SEGS + seg) * BLOCK + (int64_t)(rr % BLOCK);
-            } else {
-                if (dealt) break;
-                dealt = true;
-                gi = (int64_t)blockIdx.x * BF_THREADS + lane;
-            }
-            if (!busy && gi < m) {
-                busy = true;
-                descend = true;
-                ok = true;
-                qi = gi;
-                x = qx[gi * qs];
-                y = qy[gi * qs];
-                z = qz[gi * qs];
-                L.init(t, empty);
-                if constexpr (bf_prefills<Out>::value) out.template prefill_list<K>(gi, L, t, x, y, z);
-            }
-            want = ballot(!busy);
-            if (want != 0) seg_k++;  // the segment ran out (its indices only grow)
-        }
-        if (ballot(busy) == 0) break;
-
-        // 2. every lane advances its search to its next leaf
-        int leaf = -1;
-        bool done = false;
-        if (busy) {
-            if (descend) {
-                // greedy descent: the seed leaf
-                int ref = root;
-                while (ref >= 0) {
-                    const BvhNode4 nd = bv.nodes4[ref];
-                    const float* f = reinterpret_cast<const float*>(nd.b);
-                    const int rr[4] = {nd.r.x, nd.r.y, nd.r.z, nd.r.w};
-                    float best = INFINITY;
-                    int next = rr[0];  // slot 0 always holds a child
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        const float d = bf_child_d2(f, k, x, y, z);
-                        if (rr[k] != EMPTY4 && d < best) {
-                            best = d;
-                            next = rr[k];
-                        }
-                    }
-                    ref = next;
+                const int64_t cand = ((int64_t)(r / QUEUE_CHUNK) * QUEUE_SEGS + seg) * QUEUE_CHUNK + r % QUEUE_CHUNK;
+                if (cand < nchunks) {
+                    chunk = cand;
+                    break;
                 }
-                seed = -1 - ref;
-                leaf = seed;
-                descend = false;
-                // then best-first from the root (a single-leaf cloud is done after its seed)
-                sp = 0;
-                e = make_uint2((uint32_t)root, 0u);
-                have = root >= 0;
-            } else {
-                for (;;) {
-                    if (!have) {
-                        if (sp == 0) break;
-                        e = stk.get(--sp);
-                    }
-                    have = false;
-                    const float bound = L.bound();
-                    if (bitsf(e.y) > bound) continue;
-                    const int r = (int)e.x;
-                    if (r < 0) {
-                        if (-1 - r != seed) {
-                            leaf = -1 - r;
-                            break;
-                        }
-                        continue;
-                    }
-                    nodes++;
-                    const BvhNode4 nd = bv.nodes4[r];
-                    const float* f = reinterpret_cast<const float*>(nd.b);
-                    int c[4] = {nd.r.x, nd.r.y, nd.r.z, nd.r.w};
-                    float d[4];
-                    int nv = 0;
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        d[k] = bf_child_d2(f, k, x, y, z);
-                        if (c[k] == EMPTY4 || !(d[k] <= bound)) {
-                            c[k] = EMPTY4;
-                            d[k] = -1.0f;  // sorts last
-                        } else {
-                            nv++;
-                        }
-                    }
-                    // descending by distance: the valid children first, the nearest of them last
-                    auto cswap = [&](int a, int b) {
-                        if (d[a] < d[b]) {
-                            const float td = d[a];
-                            d[a] = d[b];
-                            d[b] = td;
-                            const int tc = c[a];
-                            c[a] = c[b];
-                            c[b] = tc;
-                        }
-                    };
-                    cswap(0, 1);
-                    cswap(2, 3);
-                    cswap(0, 2);
-                    cswap(1, 3);
-                    cswap(1, 2);
-                    // the nearest child is processed next without a trip through the stack; the
-                    // others are pushed farthest first
-                    bool full = false;
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        if (k + 1 < nv) {
-                            if (sp == stk.capacity()) {
-                                full = true;
-                            } else {
-                                stk.put(sp, make_uint2((uint32_t)c[k], fbits(d[k])));
-                                sp++;
-                            }
-                        } else if (k + 1 == nv) {
-                            e = make_uint2((uint32_t)c[k], fbits(d[k]));
-                            have = true;
-                        }
-                    }
-                    if (full) {  // the list is incomplete: the query goes to the next pass
-                        ok = false;
-                        have = false;
-                        sp = 0;
-                        break;
-                    }
-                }
-                done = leaf < 0;
+                seg_k++;  // this XCD's chunks only grow with r: all taken
             }
+        } else if (!once) {
+            chunk = blockIdx.x;
+            once = true;
         }
-        // 3. the leaves, scanned together
-        if (leaf >= 0) {
-            bf_scan_leaf<K, bf_prefills<Out>::value>(bv, leaf, x, y, z, L);
-            leaves++;
+        if (chunk < 0 || chunk >= nchunks) break;
+        const unsigned long long c0 = stats ? cycles() : 0ull;
+        const int64_t i = chunk * BF_THREADS + lane;
+        int leaves = 0, nodes = 0;
+        bool ok = true;
+        if (i < m) {  // no barrier in the search: lanes are independent
+            const float x = qx[i * qs], y = qy[i * qs], z = qz[i * qs];
+            BfList<K> L;
+            L.init(t, empty);
+            if constexpr (bf_prefills<Out>::value) out.template prefill_list<K>(i, L, t, x, y, z);
+            ok = bf_search<K, bf_prefills<Out>::value>(
+                bv, BfLdsStack{(lds_uint2*)&stk[0][lane], my_spill, BF_STACK + spill_cap}, x, y, z, L, leaves,
+                nodes);
+            out.template emit<K>(i, L, t, ok, x, y, z);
+            ok = ok && L.full();
         }
-        // 4. finished queries
-        if (done) {
-            out.template emit<K>(qi, L, t, ok, x, y, z);
-            if (ok && L.full()) n_ok++;
-            else n_bad++;
-            busy = false;
-        }
-    }
-    if (stats) {
-        int lmax = leaves, lsum = leaves, nsum = nodes, bad = n_bad, good = n_ok;
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            lmax = max(lmax, shfl_xor(lmax, o));
-            lsum += shfl_xor(lsum, o);
-            nsum += shfl_xor(nsum, o);
-            bad += shfl_xor(bad, o);
-            good += shfl_xor(good, o);
-        }
-        if (lane == 0) {
-            atomicAdd(&stats[KS_TILES], 1ull);
-            atomicAdd(&stats[KS_ATTEMPTS], (unsigned long long)lmax);
-            atomicAdd(&stats[KS_CANDIDATES], (unsigned long long)lsum * LEAF);
-            atomicAdd(&stats[KS_POPS], (unsigned long long)nsum);
-            atomicAdd(&stats[KS_REFINE], (unsigned long long)bad);
-            atomicAdd(&stats[KS_RESOLVED], (unsigned long long)good);
-            atomicAdd(&stats[KS_CY_TOTAL], cycles() - c0);
-        }
+        if (stats) bf_stats<K, Out>(stats, i < m, ok, leaves, nodes, c0);
     }
 }
 
