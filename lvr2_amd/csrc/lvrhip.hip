@@ -195,10 +195,13 @@ __global__ __launch_bounds__(CF_THREADS) void compact_scatter_kernel(const uint3
 
 // interpolateSurfaceNormals (AdaptiveKSearchSurface.tcc:312-327): the point's own normal
 // plus its ki nearest (self included), summed in ascending (d^2, index) order like the
-// reference's kSearch result, normalised, into a fresh array.  G lanes per point: the
-// kNN pass stored the ki list unordered, so each lane recomputes one neighbour's key, the
-// list is ranked in LDS (and written back sorted), then one lane sums in order.
+// reference's kSearch result, normalised, into a fresh array.  The kNN pass stored the ki list
+// unordered, so G lanes per point recompute one neighbour's key each and rank the list in LDS,
+// INTERP_POINTS points per block in turns of INTERP_THREADS / G; then one lane per point adds the
+// normals in rank order (the ordered sum is sequential: a lane per point keeps the block's lanes
+// busy where one lane of each group would leave the others idle).
 constexpr int INTERP_THREADS = 256;
+constexpr int INTERP_POINTS = 64;
 
 template <int G>
 __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
@@ -206,73 +209,62 @@ __global__ __launch_bounds__(INTERP_THREADS) void interpolate_kernel(
     const float4* __restrict__ pts4, const uint32_t* __restrict__ ki_list, int kiw, int64_t n, float* __restrict__ ox,
     float* __restrict__ oy, float* __restrict__ oz)
 {
+    constexpr int PTS = INTERP_THREADS / G;  // points ranked per turn
+    static_assert(INTERP_POINTS % PTS == 0 && INTERP_POINTS <= INTERP_THREADS, "whole turns");
     __shared__ uint64_t skey[INTERP_THREADS];
-    __shared__ uint32_t sord[INTERP_THREADS];
-    __shared__ float4 snrm[INTERP_THREADS];
+    __shared__ uint32_t sord[INTERP_POINTS][G];
+    __shared__ int snv[INTERP_POINTS];
     const int grp = threadIdx.x / G, j = threadIdx.x % G, base = grp * G;
-    const int64_t i = (int64_t)blockIdx.x * (INTERP_THREADS / G) + grp;
-    // positions with their point index in w (one 16-byte load per neighbour)
-    const float4 q = i < n ? pts4[i] : make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(~0u));
-    const bool valid = i < n && __float_as_uint(q.w) != ~0u;  // padding slots have no normal
-    uint32_t s = 0;
-    uint64_t key = ~0ull;
-    bool real = false;
-    if (valid && j < kiw) {
-        s = ki_list[(size_t)i * kiw + j];
-        if (s < n) {
-            const float4 p = pts4[s];
-            key = make_key(d2_contract(q.x, q.y, q.z, p.x, p.y, p.z), __float_as_uint(p.w));
-            real = true;
+    const int64_t first = (int64_t)blockIdx.x * INTERP_POINTS;
+    for (int turn = 0; turn < INTERP_POINTS / PTS; turn++) {
+        const int lp = turn * PTS + grp;  // the point's slot in the block
+        const int64_t i = first + lp;
+        // positions with their point index in w (one 16-byte load per neighbour)
+        const float4 q = i < n ? pts4[i] : make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(~0u));
+        const bool valid = i < n && __float_as_uint(q.w) != ~0u;  // padding slots have no normal
+        uint32_t sidx = 0;
+        uint64_t key = ~0ull;
+        bool real = false;
+        if (valid && j < kiw) {
+            sidx = ki_list[(size_t)i * kiw + j];
+            if (sidx < n) {
+                const float4 p = pts4[sidx];
+                key = make_key(d2_contract(q.x, q.y, q.z, p.x, p.y, p.z), __float_as_uint(p.w));
+                real = true;
+            }
         }
-        skey[base + j] = key;
-    }
-    // entries that name a point rank first (their keys are distinct and below ~0)
-    const int lane0 = (int)(threadIdx.x % kWave) - j;  // the group's first lane in the wave
-    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-    const int nv = __popcll((ballot(real) >> lane0) & gmask);
-    __syncthreads();
-    if (valid && j < kiw) {
-        int rank = 0;
-        for (int t = 0; t < kiw; t++) rank += skey[base + t] < key ? 1 : 0;
-        sord[base + rank] = s;
-    }
-    __syncthreads();
-    // lane j of the group loads the normal of rank j (all loads in flight at once) into LDS
-    if (j < nv) {
-        const uint32_t t = sord[base + j];
-        snrm[threadIdx.x] = make_float4(nx[t], ny[t], nz[t], 0.0f);
-    }
-    __syncthreads();
-    // the group's first lane adds them in rank order, as the reference's loop does
-    if (valid && j == 0) {
-        float x = nx[i], y = ny[i], z = nz[i];
-        const float4* sn = &snrm[base];
-        int r = 0;
-        for (; r + 4 <= nv; r += 4) {
-            const float4 a = sn[r], b = sn[r + 1], c = sn[r + 2], d = sn[r + 3];
-            x += a.x;
-            y += a.y;
-            z += a.z;
-            x += b.x;
-            y += b.y;
-            z += b.z;
-            x += c.x;
-            y += c.y;
-            z += c.z;
-            x += d.x;
-            y += d.y;
-            z += d.z;
+        skey[threadIdx.x] = key;
+        // entries that name a point rank first (their keys are distinct and below ~0)
+        const int lane0 = (int)(threadIdx.x % kWave) - j;  // the group's first lane in the wave
+        const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+        const int nv = __popcll((ballot(real) >> lane0) & gmask);
+        __syncthreads();
+        if (real) {
+            int rank = 0;
+            for (int t = 0; t < kiw; t++) rank += skey[base + t] < key ? 1 : 0;
+            sord[lp][rank] = sidx;
         }
-        for (; r < nv; r++) {
-            const float4 a = sn[r];
-            x += a.x;
-            y += a.y;
-            z += a.z;
+        if (j == 0) snv[lp] = valid ? nv : -1;
+        __syncthreads();
+    }
+    // one lane per point adds the normals in rank order, as the reference's loop does
+    if (threadIdx.x < INTERP_POINTS) {
+        const int lp = threadIdx.x;
+        const int64_t i = first + lp;
+        const int nv = snv[lp];
+        if (nv >= 0) {
+            float x = nx[i], y = ny[i], z = nz[i];
+            for (int r = 0; r < nv; r++) {
+                const uint32_t t = sord[lp][r];
+                x += nx[t];
+                y += ny[t];
+                z += nz[t];
+            }
+            normalize3(x, y, z);
+            ox[i] = x;
+            oy[i] = y;
+            oz[i] = z;
         }
-        normalize3(x, y, z);
-        ox[i] = x;
-        oy[i] = y;
-        oz[i] = z;
     }
 }
 
@@ -1300,15 +1292,16 @@ struct lvrhip_surface {
     {
         const int64_t R = npad;
         const int tok = prof.start(LVRHIP_T_INTERPOLATE, st);
+        const unsigned blocks = (unsigned)ceil_div(R, INTERP_POINTS);
         if (kiw <= 16) {
-            interpolate_kernel<16><<<(unsigned)ceil_div(R, INTERP_THREADS / 16), INTERP_THREADS, 0, st>>>(
-                nx.p, ny.p, nz.p, pts4.p, ki_list.p, kiw, R, ox, oy, oz);
+            interpolate_kernel<16><<<blocks, INTERP_THREADS, 0, st>>>(nx.p, ny.p, nz.p, pts4.p, ki_list.p, kiw, R, ox,
+                                                                    oy, oz);
         } else if (kiw <= 32) {
-            interpolate_kernel<32><<<(unsigned)ceil_div(R, INTERP_THREADS / 32), INTERP_THREADS, 0, st>>>(
-                nx.p, ny.p, nz.p, pts4.p, ki_list.p, kiw, R, ox, oy, oz);
+            interpolate_kernel<32><<<blocks, INTERP_THREADS, 0, st>>>(nx.p, ny.p, nz.p, pts4.p, ki_list.p, kiw, R, ox,
+                                                                    oy, oz);
         } else {
-            interpolate_kernel<64><<<(unsigned)ceil_div(R, INTERP_THREADS / 64), INTERP_THREADS, 0, st>>>(
-                nx.p, ny.p, nz.p, pts4.p, ki_list.p, kiw, R, ox, oy, oz);
+            interpolate_kernel<64><<<blocks, INTERP_THREADS, 0, st>>>(nx.p, ny.p, nz.p, pts4.p, ki_list.p, kiw, R, ox,
+                                                                    oy, oz);
         }
         LVRHIP_LAUNCH_CHECK();
         prof.stop(tok, st);
