@@ -845,18 +845,22 @@ struct lvrhip_surface {
     }
     void check_err() { read_err_and_counts(); }
 
+    // max_waves (with the work queue): the grid's cap, below the waves the device holds for a
+    // query list that is usually near empty (the launch then costs less than a full grid).
     template <template <int> class SinkT>
-    void tile(const QueryView& qv, const TileParams& prm_in, int P, SinkT<2> s2, SinkT<4> s4, SinkT<8> s8)
+    void tile(const QueryView& qv, const TileParams& prm_in, int P, SinkT<2> s2, SinkT<4> s4, SinkT<8> s8,
+              int64_t max_waves = 0)
     {
         TileParams prm = prm_in;
         if (use_queue) {
             d_queue.reserve(QUEUE_SEGS * QUEUE_STRIDE);
             prm.queue = d_queue.p;
         }
+        const int64_t cap = max_waves > 0 ? std::min(max_waves, resident_waves) : resident_waves;
         switch (P) {
-            case 2: launch_tile_p<2, SinkT>(view(), qv, prm, s2, st, resident_waves); break;
-            case 4: launch_tile_p<4, SinkT>(view(), qv, prm, s4, st, resident_waves); break;
-            case 8: launch_tile_p<8, SinkT>(view(), qv, prm, s8, st, resident_waves); break;
+            case 2: launch_tile_p<2, SinkT>(view(), qv, prm, s2, st, cap); break;
+            case 4: launch_tile_p<4, SinkT>(view(), qv, prm, s4, st, cap); break;
+            case 8: launch_tile_p<8, SinkT>(view(), qv, prm, s8, st, cap); break;
             default: throw Error(LVRHIP_EINVAL, "lanes per query must be 2, 4 or 8");
         }
     }
@@ -1600,8 +1604,10 @@ struct lvrhip_surface {
             return s;
         };
         const int tok = prof.start(bf ? LVRHIP_T_SDF_REDO : LVRHIP_T_SDF_KERNEL, st);
+        // after the best-first passes the list is usually empty: a wave per CU drains it
         tile<SinkSdf>(qv, prm, lanes_per_query, mk(std::integral_constant<int, 2>()),
-                      mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()));
+                      mk(std::integral_constant<int, 4>()), mk(std::integral_constant<int, 8>()),
+                      bf ? resident_waves / 16 : 0);
         prof.stop(tok, st);
         check_err();
         prof.collect();
