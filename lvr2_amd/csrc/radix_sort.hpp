@@ -195,8 +195,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const uint64_t* 
 
 // Sort n pairs by key bits [0, 64).  tmp buffers must hold n entries; hist must hold
 // rs_hist_size(n) entries.  The result ends in (keys, vals).
+// Stable sort of (key, value) pairs by key bits [begin_bit, end_bit), 8 bits per pass.
 inline void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, int64_t n,
-                             uint32_t* hist, hipStream_t st, int end_bit = 64)
+                             uint32_t* hist, hipStream_t st, int end_bit = 64, int begin_bit = 0)
 {
     if (n <= 1) return;
     const int ntiles = (int)ceil_div(n, RS_TILE);
@@ -205,7 +206,7 @@ inline void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uin
     uint64_t* kout = ktmp;
     uint32_t* vout = vtmp;
     int passes = 0;
-    for (int shift = 0; shift < end_bit; shift += 8, passes++) {
+    for (int shift = begin_bit; shift < end_bit; shift += 8, passes++) {
         rs_hist_kernel<<<ntiles, RS_THREADS, 0, st>>>(kin, n, shift, hist, ntiles);
         LVRHIP_LAUNCH_CHECK();
         scan_exclusive(hist, (int64_t)RS_RADIX * ntiles, hist + (size_t)RS_RADIX * ntiles, st);
