@@ -317,16 +317,16 @@ int lvrhip_set_profiling(lvrhip_t h, int on);
 int lvrhip_get_kernel_stats(lvrhip_t h, int64_t out[128], int reset);
 
 /* Launch shape of the kNN tile kernels (no lvr2 counterpart): lanes per query (2, 4 or 8;
- * queries per wavefront = 64 / lanes), consecutive tiles per wavefront (the adaptive rounds
- * after the first take half as many), and the radius margin carried from one tile to the
- * next.  Values <= 0 keep the current setting. */
+ * queries per wavefront = 64 / lanes), consecutive tiles per wavefront in the first adaptive
+ * round (default 12; the later rounds take one), and the radius margin carried from one tile
+ * to the next.  Values <= 0 keep the current setting. */
 int lvrhip_set_tuning(lvrhip_t h, int lanes_per_query, int tiles_per_wave, float r2_margin);
 /* ms[i], launches[i] for i < LVRHIP_T_COUNT; either may be NULL.  reset != 0 clears them. */
 int lvrhip_get_timings(lvrhip_t h, double* ms, int64_t* launches, int reset);
 
 /* Autotuner (no lvr2 counterpart): times the adaptive normals pass on a sample of the
  * cloud (every 16th run of 4096 points in Morton order) for lanes per query in {4, 8},
- * tiles per wavefront in {4, 8, 16} and radius margins in {1.15, 1.3, 1.6}, keeps the
+ * tiles per wavefront in {4, 8, 12, 16} and radius margins in {1.15, 1.3, 1.6}, keeps the
  * fastest launch shape and returns it (any output may be NULL), then keeps the faster of the
  * two retry modes (lvrhip_set_retry_mode) at that shape.  Invalidates the normals. */
 int lvrhip_autotune(lvrhip_t h, int* lanes_per_query, int* tiles_per_wave, float* r2_margin);

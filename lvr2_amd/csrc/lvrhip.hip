@@ -666,7 +666,7 @@ struct lvrhip_surface {
     float flip[3] = {1e6f, 1e6f, 1e6f};
     bool scan_flips = false;
     int lanes_per_query = 8;
-    int tiles_per_wave = 8;
+    int tiles_per_wave = 12;
     float r2_margin = 1.15f;
     bool merge_retry = false;  // rounds 2..5 in one four-target search
     // launch shape of the signed-distance searches: lattice vertices are far fewer than points
@@ -1066,8 +1066,14 @@ struct lvrhip_surface {
         for (int r = 1; r <= last; r++) {
             const int t0 = clamp_k((int64_t)kn << r);
             TileParams prm = params(t0, r == 1 ? kiw : 0, 0);
-            // the later rounds' query lists are sparse: shorter runs balance their costlier tiles
-            if (r > 1) prm.tiles_per_wave = std::max(1, tiles_per_wave / 2);
+            // the later rounds' query lists are sparse: a run's consecutive tiles lie far apart, so
+            // its shared leaf list does not pay, and one tile per run balances their costlier tiles;
+            // round 1 keeps at least two runs per resident wave (a small cloud has few tiles)
+            if (r > 1)
+                prm.tiles_per_wave = 1;
+            else
+                prm.tiles_per_wave = (int)std::clamp<int64_t>(
+                    ceil_div(nq0, kWave / lanes_per_query) / (2 * resident_waves), 1, tiles_per_wave);
             // each round's k-th d^2 per point sizes the next round's first radius: k doubles, and
             // on a surface the count grows like d^2 (x 1.15 of margin; the tuned r2_margin only
             // sizes the radius carried from tile to tile)
@@ -1456,7 +1462,7 @@ struct lvrhip_surface {
         LVRHIP_CHECK(hipEventCreate(&a));
         LVRHIP_CHECK(hipEventCreate(&b));
         const int Ps[2] = {4, 8};
-        const int Ts[3] = {4, 8, 16};
+        const int Ts[4] = {4, 8, 12, 16};
         const float Ms[3] = {1.15f, 1.3f, 1.6f};
         float best = INFINITY;
         int bp = lanes_per_query, bt = tiles_per_wave;

@@ -205,7 +205,7 @@ def test_autotune_keeps_results(project):
     stats = s.normalsStats()
     best = s.autotune()
     assert best["lanes_per_query"] in (4, 8)
-    assert best["tiles_per_wave"] in (4, 8, 16)
+    assert best["tiles_per_wave"] in (4, 8, 12, 16)
     assert best == s.tuning()
     s.calculateNormals()
     np.testing.assert_array_equal(s.normalsStats(), stats)
