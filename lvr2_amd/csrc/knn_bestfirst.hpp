@@ -142,21 +142,22 @@ __device__ __forceinline__ void bf_scan_leaf(const BvhView& bv, int leaf, float 
         // padding slots have NaN coordinates: never a neighbour
         c[j] = (d == d && key < w) ? key : ~0ull;
     }
+    // Keys are unique (they end in the point's index), so each round takes the smallest key above
+    // the previous round's instead of clearing the taken slot: the same sequence of insertions.
+    uint64_t lo = 0;
     for (int round = 0; round < LEAF; round++) {
-        uint64_t best = c[0];
+        uint64_t best = ~0ull;
         int bj = 0;
 #pragma unroll
-        for (int j = 1; j < LEAF; j++) {
-            if (c[j] < best) {
+        for (int j = 0; j < LEAF; j++) {
+            if (c[j] >= lo && c[j] < best) {
                 best = c[j];
                 bj = j;
             }
         }
         if (!(best < L.worst())) break;  // the rest cannot enter (~0 marks none)
         if (!kDedup || !L.contains(best)) L.insert(best, (uint32_t)(leaf * LEAF + bj));
-#pragma unroll
-        for (int j = 0; j < LEAF; j++)
-            if (j == bj) c[j] = ~0ull;
+        lo = best + 1;  // best < worst <= ~0: no wrap
     }
 }
 
