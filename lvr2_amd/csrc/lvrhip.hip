@@ -1146,6 +1146,7 @@ struct lvrhip_surface {
         TileParams prm = params(clamp_k((int64_t)kn << 5), clamp_k((int64_t)kn << 4), 1);
         prm.t2 = clamp_k((int64_t)kn << 3);
         prm.t3 = clamp_k((int64_t)kn << 2);
+        prm.tiles_per_wave = 1;  // sparse lists, as in the rounds searched one at a time
         prm.stats = kstats_ptr(5);
         uint32_t* redo = s_flags.p;
         {
@@ -1175,6 +1176,7 @@ struct lvrhip_surface {
             compact_flags(a_sel.p, nq0, nq_dev, qsel, a_list.p, counter(6));
             const int t0 = clamp_k((int64_t)kn << r);
             TileParams pr = params(t0, 0, 1);
+            pr.tiles_per_wave = 1;
             pr.stats = kstats_ptr(r - 1);
             QueryView qv{px.p, py.p, pz.p, a_list.p, (uint32_t)nq0, counter(6)};
             auto mk = [&](auto tag) {
